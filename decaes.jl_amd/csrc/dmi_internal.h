@@ -1,0 +1,78 @@
+// Internal host/device interface between the C ABI (dmi_api.cpp) and the gfx950 kernels (dmi_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace dmi {
+
+constexpr int kWave = 64;
+constexpr int kMaxT2 = 64;        // one lane per T2 column
+constexpr int kMaxEcho = 124;     // lane-per-state EPG window (ETL/2 + 2 states)
+constexpr int kMaxAngles = 64;    // one lane per flip-angle grid point
+constexpr int kTikhCache = 8;     // NNLSTikhonovRegProblemCache size (src/lsqnonneg.jl:396)
+constexpr int kLcurveCap = 128;   // L-curve cache capacity per voxel (reference starts at 64 and grows)
+
+enum StatusBits : int {
+    kStatusLcurveOverflow = 1,
+    kStatusNnlsMaxIter = 2,
+};
+
+enum VoxelMode : int { kModeSearch = 0, kModeB1 = 1, kModeFixed = 2 };
+
+struct GridParams {
+    int m, n, K;
+    int general;          // 1: RefConAngle != 180 (general refocusing train)
+    double beta;          // RefConAngle
+    double E1;            // exp(-(TE/2)/T1)
+    const double *E2;     // n: exp(-(TE/2)/T2_j)
+    const double *alphas; // K
+    double *As;           // K x m x n, row-major per angle
+    double *dAs;          // K x m x n or nullptr
+};
+
+struct T2mapParams {
+    int m, n, K, mineval, reg, mode, general;
+    int lda, mw;          // LDS workspace: mw = m + n rows, leading dimension lda
+    int save_resnorm, save_decaycurve, save_regparam, save_basis;
+    int64_t V;            // voxels in the volume
+    int64_t nvox;         // voxels above threshold
+    double E1, beta, chi2factor, noiselevel, setflip;
+    const double *image;  // (V, m) column-major
+    const int64_t *vox;   // nvox linear indices
+    unsigned int *counter;
+    const double *E2;     // n
+    const double *logT2;  // n
+    const double *alphas; // K
+    const double *As;     // K x m x n row-major grid bases
+    const double *dAs;    // their d/dalpha
+    const double *Afixed; // m x n row-major (SetFlipAngle)
+    const double *alpha_in;
+    double *scratch;      // per-workgroup slots
+    int64_t slot_stride;  // doubles per slot
+    double *dist, *gdn, *ggm, *gva, *fnr, *snr, *alpha, *resnorm, *decaycurve, *mu, *chi2f, *decaybasis;
+    unsigned long long *counters; // [0] fp64 flops, [1] nnls solves
+    int *status;
+};
+
+struct T2partParams {
+    int n;
+    int64_t V;
+    int sp0, sp1, mp0, mp1;   // inclusive 0-based window ranges
+    const double *logT2;      // n
+    const double *weights;    // n or nullptr (sigmoid)
+    const double *dist;
+    double *sfr, *sgm, *mfr, *mgm;
+};
+
+size_t t2map_lds_bytes(int m, int n, int lda, int mw);
+int64_t t2map_slot_doubles(int m, int n);
+
+hipError_t launch_grid_basis(const GridParams &p, hipStream_t s);
+hipError_t launch_select(const double *image, int64_t V, double threshold, int64_t *vox, unsigned int *count,
+                         hipStream_t s);
+hipError_t launch_t2map(const T2mapParams &p, int grid, size_t lds, hipStream_t s);
+hipError_t launch_t2part(const T2partParams &p, hipStream_t s);
+hipError_t t2map_occupancy(int reg, int *blocks_per_cu, size_t lds);
+
+}  // namespace dmi

@@ -1,0 +1,1572 @@
+// gfx950 kernels of the voxelwise T2mapSEcorr / T2partSEcorr path.
+//
+// Execution model: one 64-lane wavefront per voxel, one wave per workgroup, a persistent grid pulling
+// voxel indices from a device counter. Per-voxel dense state (the Lawson-Hanson workspace, right-hand
+// side, duals, Tikhonov cache keys) lives in LDS; the per-voxel EPG basis, cached Tikhonov solutions and
+// the L-curve caches live in a per-workgroup global scratch slot that stays resident in L2. Lanes take the
+// natural parallel dimension of each step: columns for Householder applications and duals, rows for
+// reflector construction, residuals and column shifts, phase states for the EPG recursion, grid angles for
+// the flip-angle surrogate. Scalar control flow (active-set loop, Brent, bisection search, L-curve) is
+// wave-uniform: every lane computes identical values from identical inputs, and every reduction is a
+// butterfly whose partner lanes hold identical partials, finished with a readfirstlane broadcast.
+//
+// Arithmetic follows the reference statement by statement, with fma() where DECAES writes muladd or sits
+// under @muladd (src/NNLS.jl:47, src/EPGdecaycurve.jl:858-1024); built with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+
+#include "dmi_internal.h"
+
+namespace dmi {
+namespace {
+
+constexpr double kDeg2Rad = 3.141592653589793 / 180.0;  // Julia deg2rad: x * (Float64(pi)/180)
+constexpr double kPi180Hi = 0.017453292519943295;      // pi/180 as a double-double
+constexpr double kPi180Lo = 2.9486522708701687e-19;
+constexpr double kGolden = 1.618033988749895;          // Float64(Base.MathConstants.golden)
+constexpr double kSqrtEps = 1.4901161193847656e-8;
+constexpr double kEps = 2.220446049250313e-16;
+
+// ------------------------------------------------------------------------------------------------
+// Wave utilities
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ double rfl(double v) {
+    unsigned long long u = __double_as_longlong(v);
+    unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(u & 0xffffffffull));
+    unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double readlane(double v, int l) {
+    unsigned long long u = __double_as_longlong(v);
+    unsigned lo = __builtin_amdgcn_readlane((unsigned)(u & 0xffffffffull), l);
+    unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return rfl(v);
+}
+__device__ __forceinline__ double wmaxd(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return rfl(v);
+}
+__device__ __forceinline__ double wmind(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
+    return rfl(v);
+}
+// (value desc, index asc) arg-max and (value asc, index asc) arg-min.
+__device__ __forceinline__ void wargmax(double &v, int &i) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        double ov = __shfl_xor(v, o, kWave);
+        int oi = __shfl_xor(i, o, kWave);
+        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+    v = rfl(v);
+    i = rfl(i);
+}
+__device__ __forceinline__ void wargmin(double &v, int &i) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        double ov = __shfl_xor(v, o, kWave);
+        int oi = __shfl_xor(i, o, kWave);
+        if (ov < v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+    v = rfl(v);
+    i = rfl(i);
+}
+__device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ int first_set(unsigned long long m) { return m ? __ffsll((long long)m) - 1 : -1; }
+__device__ __forceinline__ int last_set(unsigned long long m) { return m ? 63 - __clzll((long long)m) : -1; }
+
+__device__ __forceinline__ double jl_max(double a, double b) { return (isnan(a) || isnan(b)) ? NAN : (a > b ? a : b); }
+__device__ __forceinline__ double jl_min(double a, double b) { return (isnan(a) || isnan(b)) ? NAN : (a < b ? a : b); }
+__device__ __forceinline__ double jl_sign(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : x); }
+__device__ __forceinline__ bool jl_isapprox(double x, double y) {
+    if (x == y) return true;
+    if (!isfinite(x) || !isfinite(y)) return false;
+    return fabs(x - y) <= kSqrtEps * fmax(fabs(x), fabs(y));
+}
+__device__ __forceinline__ bool jl_isless(double a, double b) {
+    if (isnan(a)) return false;
+    if (isnan(b)) return true;
+    return a < b;
+}
+__device__ __forceinline__ double norm2(double a, double b) {
+    if (isnan(a) || isnan(b)) return NAN;
+    if (isinf(a) || isinf(b)) return INFINITY;
+    return sqrt(a * a + b * b);
+}
+
+// sind / cosd with exact degree reduction (Julia Base.Math) and a double-double pi/180 product.
+__device__ __forceinline__ double sin_dd(double x) {
+    double hi = x * kPi180Hi, lo = fma(x, kPi180Hi, -hi) + x * kPi180Lo, s, c;
+    sincos(hi, &s, &c);
+    return fma(c, lo, s);
+}
+__device__ __forceinline__ double cos_dd(double x) {
+    double hi = x * kPi180Hi, lo = fma(x, kPi180Hi, -hi) + x * kPi180Lo, s, c;
+    sincos(hi, &s, &c);
+    return fma(-s, lo, c);
+}
+__device__ double dsind(double x) {
+    if (isinf(x) || isnan(x)) return NAN;
+    double rx = copysign(fmod(x, 360.0), x), arx = fabs(rx);
+    if (rx == 0) return rx;
+    if (arx < 45.0) return sin_dd(rx);
+    if (arx <= 135.0) return copysign(cos_dd(90.0 - arx), rx);
+    if (arx == 180.0) return copysign(0.0, rx);
+    if (arx < 225.0) return sin_dd((180.0 - arx) * jl_sign(rx));
+    if (arx <= 315.0) return -copysign(cos_dd(270.0 - arx), rx);
+    return sin_dd(rx - copysign(360.0, rx));
+}
+__device__ double dcosd(double x) {
+    if (isinf(x) || isnan(x)) return NAN;
+    double rx = fabs(fmod(x, 360.0));
+    if (rx <= 45.0) return cos_dd(rx);
+    if (rx < 135.0) return sin_dd(90.0 - rx);
+    if (rx <= 225.0) return -cos_dd(180.0 - rx);
+    if (rx < 315.0) return sin_dd(rx - 270.0);
+    return cos_dd(360.0 - rx);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Numbers for the EPG recursion: double, or a forward-mode dual (value, d/dalpha) as ForwardDiff.Dual
+// computes in EPGJacobianFunctor (src/EPGdecaycurve.jl:224-248).
+// ------------------------------------------------------------------------------------------------
+struct Dual {
+    double v, d;
+};
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a) { return {-a.v, -a.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, double s) { return {a.v * s, a.d * s}; }
+__device__ __forceinline__ Dual operator/(Dual a, double s) { return {a.v / s, a.d / s}; }
+__device__ __forceinline__ Dual tfma(Dual a, Dual b, Dual c) { return {fma(a.v, b.v, c.v), a.d * b.v + a.v * b.d + c.d}; }
+__device__ __forceinline__ double tfma(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ Dual tabs(Dual a) { return signbit(a.v) ? -a : a; }
+__device__ __forceinline__ double tabs(double a) { return fabs(a); }
+__device__ __forceinline__ void tsincos(Dual x, Dual *s, Dual *c) {
+    double sv, cv;
+    sincos(x.v, &sv, &cv);
+    *s = {sv, cv * x.d};
+    *c = {cv, -sv * x.d};
+}
+__device__ __forceinline__ void tsincos(double x, double *s, double *c) { sincos(x, s, c); }
+template <typename T> struct Num;
+template <> struct Num<double> {
+    static __device__ __forceinline__ double c(double v) { return v; }
+    static __device__ __forceinline__ double var(double v, double) { return v; }
+    static __device__ __forceinline__ void put(double x, double *v, double *, int64_t o) { v[o] = x; }
+};
+template <> struct Num<Dual> {
+    static __device__ __forceinline__ Dual c(double v) { return {v, 0.0}; }
+    static __device__ __forceinline__ Dual var(double v, double d) { return {v, d}; }
+    static __device__ __forceinline__ void put(Dual x, double *v, double *d, int64_t o) {
+        v[o] = x.v;
+        if (d) d[o] = x.d;
+    }
+};
+template <int LPC> __device__ __forceinline__ double shup(double a) { return __shfl_up(a, 1, LPC); }
+template <int LPC> __device__ __forceinline__ double shdn(double a) { return __shfl_down(a, 1, LPC); }
+template <int LPC> __device__ __forceinline__ Dual shup(Dual a) { return {__shfl_up(a.v, 1, LPC), __shfl_up(a.d, 1, LPC)}; }
+template <int LPC> __device__ __forceinline__ Dual shdn(Dual a) { return {__shfl_down(a.v, 1, LPC), __shfl_down(a.d, 1, LPC)}; }
+
+// One EPG decay curve per LPC-lane group, lane s = phase state s+1. Each echo maps state k = (F, Fbar, Z)
+// to F_{k+1}, Fbar_{k-1} and Z_k; the state-1 Fbar result becomes the new F_1 and is the echo. The
+// reference's split loops drop states that cannot reach an echo before the train ends; a window of
+// ETL/2 + 2 lanes reproduces them exactly (states outside it never flow back into it in time).
+//
+// Constant-flip train: EPGConstantFlipAngleOptions, DualFlat recursion (:948-1028) + scaling (:936-946).
+template <int LPC, typename T>
+__device__ void epg_const(double alpha_deg, double E1, double E2, int ETL, int s, bool store, double *out_v,
+                          double *out_d, int64_t ostride) {
+    using N = Num<T>;
+    const T al = N::var(alpha_deg * kDeg2Rad, kDeg2Rad);
+    T sn, cs;
+    tsincos(al, &sn, &cs);
+    const T m0 = N::var(dsind(alpha_deg / 2), dcosd(alpha_deg / 2) * kDeg2Rad * 0.5);
+    const double E2h = (E2 * E2) / 2, E1E2 = E1 * E2, E1sq = E1 * E1;
+    const T a = N::c(E2h), b = cs * E2h, c = sn * E1E2, d = cs * E1sq, cp = (-c) / 2.0, mc = -c;
+    T F = N::c(s == 0 ? 1.0 : 0.0), Fb = N::c(0.0), Z = N::c(0.0);
+    for (int i = 0; i < ETL; ++i) {
+        const T C = F + Fb, S = F - Fb;
+        T echo;
+        if (i < ETL - 1) {
+            const T Cp = a * C, Sp = b * S;
+            const T Fr = tfma(c, Z, Cp + Sp), Fbr = tfma(mc, Z, Cp - Sp), Zr = tfma(cp, S, d * Z);
+            const T Fup = shup<LPC>(Fr), Fbdn = shdn<LPC>(Fbr);
+            echo = Fbr;
+            F = (s == 0) ? Fbr : Fup;
+            Fb = (s == LPC - 1) ? N::c(0.0) : Fbdn;
+            Z = Zr;
+        } else {
+            echo = tfma(mc, Z, tfma(a, C, -(b * S)));
+        }
+        if (store && s == 0) N::put(tabs(m0 * echo), out_v, out_d, i * ostride);
+    }
+}
+
+// General refocusing control angle: EPGOptions, DualVector recursion (:722-818). Dot products follow
+// StaticArrays' order (a1 b1 + a2 b2) + a3 b3 without fma; first pulse A*180, later pulses A*beta.
+template <typename T> __device__ __forceinline__ T dot3(T x0, T x1, T x2, T y0, T y1, T y2) {
+    return (x0 * y0 + x1 * y1) + x2 * y2;
+}
+template <int LPC, typename T>
+__device__ void epg_general(double alpha_deg, double beta, double E1, double E2, int ETL, int s, bool store,
+                            double *out_v, double *out_d, int64_t ostride) {
+    using N = Num<T>;
+    const T alpha = N::var(alpha_deg, 1.0);
+    const T A = alpha / 180.0;
+    const T a1r = (A * 180.0) * kDeg2Rad, air = (A * beta) * kDeg2Rad;
+    T s_h, c_h, sinai, cosai;
+    tsincos(a1r / 2.0, &s_h, &c_h);
+    tsincos(air, &sinai, &cosai);
+    const T s2 = s_h * s_h, c2 = c_h * c_h, sina1 = (s_h * 2.0) * c_h;
+    const T c2i = (N::c(1.0) + cosai) / 2.0, s2i = N::c(1.0) - c2i;
+    const double E2sq = E2 * E2, E1E2 = E1 * E2, E1sq = E1 * E1;
+    const T a1 = c2 * E2sq, b1 = s2 * E2sq, c1 = sina1 * E1E2;
+    const T ai = c2i * E2sq, bi = s2i * E2sq, ci = sinai * E1E2, di = cosai * E1sq;
+    const T zc0 = (-ci) / 2.0, zc1 = ci / 2.0, mci = -ci;
+    const T m0 = s_h;
+    T F = N::c(0.0), Fb = N::c(0.0), Z = N::c(0.0);
+    if (s == 0) { F = b1 * m0; Z = ((-c1) * m0) / 2.0; }
+    if (s == 1) F = a1 * m0;
+    if (store && s == 0) N::put(tabs(b1 * m0), out_v, out_d, 0);
+    for (int i = 1; i < ETL; ++i) {
+        const T FM = dot3(ai, bi, ci, F, Fb, Z), FbM = dot3(bi, ai, mci, F, Fb, Z), ZM = dot3(zc0, zc1, di, F, Fb, Z);
+        if (store && s == 0) N::put(tabs(FbM), out_v, out_d, i * ostride);
+        const T Fup = shup<LPC>(FM), Fbdn = shdn<LPC>(FbM);
+        F = (s == 0) ? FbM : Fup;
+        Fb = (s == LPC - 1) ? N::c(0.0) : Fbdn;
+        Z = ZM;
+    }
+}
+
+template <int LPC, typename T>
+__device__ __forceinline__ void epg_curve(bool general, double alpha, double beta, double E1, double E2, int ETL,
+                                          int s, bool store, double *out_v, double *out_d, int64_t ostride) {
+    if (general) epg_general<LPC, T>(alpha, beta, E1, E2, ETL, s, store, out_v, out_d, ostride);
+    else epg_const<LPC, T>(alpha, E1, E2, ETL, s, store, out_v, out_d, ostride);
+}
+
+// Grid bases A_k and dA_k/dalpha for every flip angle of the search (EPGBasisSetEnsemble with
+// ∇epg_decay_basis!, src/T2mapSEcorr.jl:339-371). Block = (angle k, T2 pair); row-major K x m x n.
+template <int LPC>
+__global__ __launch_bounds__(64) void grid_basis_kernel(GridParams p) {
+    constexpr int per = kWave / LPC;
+    const int k = blockIdx.x, lane = lane_id(), h = lane / LPC, s = lane % LPC;
+    const int j = blockIdx.y * per + h;
+    const bool valid = j < p.n;
+    const int jj = valid ? j : p.n - 1;
+    const int64_t base = (int64_t)k * p.m * p.n + jj;
+    if (p.dAs)
+        epg_curve<LPC, Dual>(p.general, p.alphas[k], p.beta, p.E1, p.E2[jj], p.m, s, valid, p.As + base, p.dAs + base, p.n);
+    else
+        epg_curve<LPC, double>(p.general, p.alphas[k], p.beta, p.E1, p.E2[jj], p.m, s, valid, p.As + base, nullptr, p.n);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Voxel selection: indices with image[I, 1] > Threshold (src/T2mapSEcorr.jl:177). Order is irrelevant:
+// voxels are independent and results are written at their own index.
+// ------------------------------------------------------------------------------------------------
+__global__ void select_kernel(const double *image, int64_t V, double thr, int64_t *vox, unsigned int *count) {
+    for (int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I < V; I += (int64_t)gridDim.x * blockDim.x)
+        if (image[I] > thr) vox[atomicAdd(count, 1u)] = I;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Per-voxel wave state
+// ------------------------------------------------------------------------------------------------
+struct Wave {
+    // LDS
+    double *A, *b, *zz, *x, *w, *xf, *b0, *cmu, *crn, *csemi;
+    int *idx, *diag, *stk;
+    int lda, mw, m, n, L;
+    // per-workgroup global scratch
+    double *A0s, *cx, *lc;
+    const double *A0;   // basis used by the regularization solves (A0s or the fixed basis)
+    int ccur;
+    unsigned long long flops, solves;
+    int status;
+
+    __device__ __forceinline__ double &Aat(int i, int j) { return A[i * lda + j]; }
+
+    // -------------------------------- Lawson-Hanson NNLS (src/NNLS.jl) --------------------------------
+    // construct_apply_householder! (:259-336), 0-based: pivot row ip, column jp, rows [0, mr).
+    __device__ double construct_apply_householder(int ip, int jp, int mr) {
+        if (ip >= mr) return 0.0;
+        double alpha = rfl(Aat(ip, jp));
+        double part = 0.0;
+        for (int i = ip + L; i < mr; i += kWave) { double a = Aat(i, jp); part = fma(a, a, part); }
+        double xnorm = sqrt(wsum(part));
+        flops += 2ull * (mr - ip) + 8;
+        if (xnorm == 0) return -1.0;
+        double beta = copysign(xnorm, alpha);
+        alpha = alpha + beta;
+        double tau = alpha / beta;
+        double bip = rfl(b[ip]);
+        part = 0.0;
+        for (int i = ip + 1 + L; i < mr; i += kWave) part = fma(b[i], Aat(i, jp) / alpha, part);
+        double sm = bip + wsum(part);
+        sm *= -tau;
+        double A1 = -beta, b1 = bip + sm;
+        flops += 3ull * (mr - ip);
+        if (!(b1 / A1 > 0)) return -1.0;
+        __syncthreads();
+        if (ip < mr - 1) {
+            if (ip != jp) {
+                for (int i = L; i < mr; i += kWave) {
+                    if (i < ip) {
+                        double t = Aat(i, ip); Aat(i, ip) = Aat(i, jp); Aat(i, jp) = t;
+                    } else if (i == ip) {
+                        b[ip] = b1;
+                        double t = Aat(ip, ip); Aat(ip, ip) = A1; Aat(ip, jp) = t;
+                    } else {
+                        double Aij = Aat(i, jp) / alpha, t = Aat(i, ip);
+                        Aat(i, ip) = Aij; Aat(i, jp) = t;
+                        b[i] = fma(sm, Aij, b[i]);
+                    }
+                }
+            } else {
+                for (int i = L; i < mr; i += kWave) {
+                    if (i == ip) { b[ip] = b1; Aat(ip, ip) = A1; }
+                    else if (i > ip) { double Aii = Aat(i, ip) / alpha; Aat(i, ip) = Aii; b[i] = fma(sm, Aii, b[i]); }
+                }
+            }
+            flops += 3ull * (mr - ip);
+        } else {
+            tau = 0.0;
+            if (ip != jp)
+                for (int i = L; i < mr; i += kWave) { double t = Aat(i, ip); Aat(i, ip) = Aat(i, jp); Aat(i, jp) = t; }
+        }
+        __syncthreads();
+        return tau;
+    }
+
+    // apply_householder_dual! (:379-436): columns j > j1 (one per lane), rows [j1, m1).
+    __device__ void apply_householder_dual(double tau, int j1, int m1) {
+        if (j1 >= m1 - 1) return;
+        const int j = L;
+        if (j > j1 && j < n) {
+            double sm = Aat(j1, j);
+            for (int i = j1 + 1; i < m1; ++i) sm = fma(Aat(i, j), Aat(i, j1), sm);
+            sm *= -tau;
+            Aat(j1, j) = Aat(j1, j) + sm;
+            double wj = 0.0;
+            for (int i = j1 + 1; i < m1; ++i) {
+                double Aij = fma(sm, Aat(i, j1), Aat(i, j));
+                wj = fma(Aij, b[i], wj);
+                Aat(i, j) = Aij;
+            }
+            w[j] = wj;
+        }
+        flops += 6ull * (m1 - j1) * (n - j1 - 1);
+        __syncthreads();
+    }
+
+    // compute_dual! (:441-470): w_j = sum_{i >= j1} A_ij b_i for j >= j1, rows [j1, m1).
+    __device__ void compute_dual(int j1, int m1) {
+        const int j = L;
+        if (j >= j1 && j < n) {
+            double sm = 0.0;
+            for (int i = j1; i < m1; ++i) sm = fma(Aat(i, j), b[i], sm);
+            w[j] = sm;
+        }
+        flops += 2ull * (m1 - j1) * (n - j1);
+        __syncthreads();
+    }
+
+    // solve_triangular_system! (:506-524) on R = A[0:k, 0:k]; lane i holds z_i in a register.
+    __device__ double tri_solve(int k, double zr) {
+        for (int j = k - 1; j >= 0; --j) {
+            double zj = readlane(zr, j);
+            double zi = -zj / rfl(Aat(j, j));
+            if (L < j) zr = fma(Aat(L, j), zi, zr);
+            else if (L == j) zr = -zi;
+        }
+        flops += (unsigned long long)k * k;
+        return zr;
+    }
+
+    // Remove coefficient imv from set P by Givens rotations and column shifts, then drop any further
+    // nonpositive coefficients (:729-779 / :965-1015).
+    __device__ void nnls_delete(int &imv, int &nsetp, int mrows) {
+        for (;;) {
+            if (L == 0) x[idx[imv]] = 0.0;
+            __syncthreads();
+            if (imv != nsetp - 1) {
+                for (int i = imv + 1; i <= nsetp - 1; ++i) {
+                    double aa = rfl(Aat(i - 1, i)), bb = rfl(Aat(i, i));
+                    double sig = hypot(aa, bb), cc = aa / sig, ss = bb / sig;
+                    __syncthreads();
+                    const int j = L;
+                    if (j < n) {
+                        if (j == i) { Aat(i - 1, i) = sig; Aat(i, i) = 0.0; }
+                        else {
+                            double u = Aat(i - 1, j), v = Aat(i, j);
+                            Aat(i - 1, j) = fma(cc, u, ss * v);
+                            Aat(i, j) = fma(-ss, u, cc * v);
+                        }
+                    }
+                    if (L == 0) {
+                        double u = b[i - 1], v = b[i];
+                        b[i - 1] = fma(cc, u, ss * v);
+                        b[i] = fma(-ss, u, cc * v);
+                    }
+                    flops += 6ull * (n + 1);
+                    __syncthreads();
+                }
+                for (int r = L; r < mrows; r += kWave) {
+                    double t = Aat(r, imv);
+                    for (int j = imv; j < nsetp - 1; ++j) Aat(r, j) = Aat(r, j + 1);
+                    Aat(r, nsetp - 1) = t;
+                }
+                if (L == 0) {
+                    int t = idx[imv];
+                    for (int j = imv; j < nsetp - 1; ++j) idx[j] = idx[j + 1];
+                    idx[nsetp - 1] = t;
+                }
+                __syncthreads();
+            }
+            nsetp -= 1;
+            bool bad = (L < nsetp) && (x[idx[L]] <= 0);
+            int f = first_set(ballot(bad));
+            if (f < 0) break;
+            imv = f;
+        }
+    }
+
+    // unsafe_nnls! (:605-825) and its Tikhonov twin unsafe_nnls!(work, λ) (:827-1061). Workspace rows
+    // [0, M) with M = m (plain) or m + n (Tikhonov rows appended lazily when a column first enters).
+    __device__ void nnls_main(bool tikh, double lam, int &nsetp_out, double &rnorm_out) {
+        const int M = tikh ? m + n : m;
+        int mr = m;  // current number of active rows (the reference's m)
+        int nsetp = 0, iter = 0;
+        const int max_iter = 3 * n;
+        bool terminated = false;
+        for (;;) {
+            if (tikh ? (nsetp >= n) : (nsetp >= n || nsetp >= mr)) break;
+            int jmax = -1;
+            double tau = 0.0;
+            bool term = false;
+            for (;;) {
+                double wv = (L >= nsetp && L < n) ? w[L] : 0.0;
+                if (!(wv > 0)) wv = 0.0;
+                int wi = (wv > 0) ? L : INT_MAX;
+                wargmax(wv, wi);
+                if (!(wv > 0)) { term = true; break; }
+                jmax = wi;
+                bool newdiag = tikh && !diag[idx[jmax]];
+                if (newdiag) {
+                    if (L == 0) Aat(mr, jmax) = lam;
+                    __syncthreads();
+                }
+                const int mrh = tikh ? min(mr + 1, M) : mr;
+                tau = construct_apply_householder(nsetp, jmax, mrh);
+                if (tau >= 0) break;
+                if (L == 0) {
+                    w[jmax] = 0.0;
+                    if (tikh && mr < M) Aat(mr, jmax) = 0.0;
+                }
+                __syncthreads();
+            }
+            if (term) break;
+            if (tikh && !rfl(diag[idx[jmax]])) {
+                mr = min(mr + 1, M);
+                __syncthreads();
+                if (L == 0) diag[idx[jmax]] = 1;
+            }
+            nsetp += 1;
+            __syncthreads();
+            if (L == 0) { int t = idx[nsetp - 1]; idx[nsetp - 1] = idx[jmax]; idx[jmax] = t; }
+            __syncthreads();
+            if (nsetp < n) apply_householder_dual(tau, nsetp - 1, mr);
+            for (int i = nsetp + L; i < mr; i += kWave) Aat(i, nsetp - 1) = 0.0;
+            if (L == 0) w[nsetp - 1] = 0.0;
+            __syncthreads();
+            double zr = (L < nsetp) ? b[L] : 0.0;
+            zr = tri_solve(nsetp, zr);
+            bool dual_flag = false;
+            for (;;) {
+                iter += 1;
+                if (iter > max_iter) { status |= kStatusNnlsMaxIter; terminated = true; break; }
+                double t = INFINITY;
+                int ti = INT_MAX;
+                if (L < nsetp && zr <= 0) {
+                    double xi = x[idx[L]];
+                    double tt = -xi / (zr - xi);
+                    if (tt < 2.0) { t = tt; ti = L; }
+                }
+                wargmin(t, ti);
+                if (ti == INT_MAX) break;  // alpha == 2: all feasible
+                dual_flag = true;
+                const double alpha = t;
+                int imv = ti;
+                if (L < nsetp) {
+                    int ix = idx[L];
+                    x[ix] = fma(alpha, zr - x[ix], x[ix]);
+                }
+                __syncthreads();
+                nnls_delete(imv, nsetp, mr);
+                zr = (L < nsetp) ? b[L] : 0.0;
+                zr = tri_solve(nsetp, zr);
+            }
+            if (terminated) break;
+            if (dual_flag) compute_dual(nsetp, mr);
+            if (L < nsetp) x[idx[L]] = zr;
+            __syncthreads();
+        }
+        const int Mlim = tikh ? M : mr;
+        double part = 0.0;
+        for (int i = nsetp + L; i < Mlim; i += kWave) { double bi = b[i]; part = fma(bi, bi, part); }
+        rnorm_out = sqrt(wsum(part));
+        nsetp_out = nsetp;
+    }
+
+    // NNLSProblem solve! with the reference warm start (src/lsqnonneg.jl:30-84), or the Tikhonov form
+    // (:86-164) when tikh. A0g: global row-major m x n basis; right-hand side b0 in LDS.
+    __device__ void solve(const double *__restrict__ A0g, bool tikh, double mu, int &nsetp, double &rnorm) {
+        const int M = tikh ? m + n : m;
+        double pd = 0.0;
+        for (int i = L; i < m; i += kWave) { double a = A0g[i * n + n - 1]; pd += a * a; }
+        double den = wsum(pd);
+        if (tikh) den += mu * mu;
+        double px = 0.0;
+        for (int i = L; i < m; i += kWave) px += (A0g[i * n + n - 1] / den) * b0[i];
+        double xj = wsum(px);
+        for (int i = L; i < m; i += kWave) zz[i] = b0[i] - A0g[i * n + n - 1] * xj;
+        __syncthreads();
+        bool pos = false;
+        if (L < n) {
+            double wj = 0.0;
+            for (int i = 0; i < m; ++i) {
+                double a = A0g[i * n + L];
+                wj += a * zz[i];
+                Aat(i, L) = a;
+            }
+            if (L < n - 1) { w[L] = wj; pos = !(wj <= 0); }
+            x[L] = 0.0;
+            idx[L] = L;
+            diag[L] = 0;
+            if (tikh)
+                for (int i = m; i < M; ++i) Aat(i, L) = 0.0;
+        }
+        unsigned long long bal = ballot(pos);
+        if (L == n - 1) w[L] = (bal == 0) ? 1.0 : 0.0;
+        for (int i = L; i < m; i += kWave) b[i] = b0[i];
+        if (tikh)
+            for (int i = m + L; i < M; i += kWave) b[i] = 0.0;
+        flops += 2ull * m * n + 6ull * m;
+        solves += 1;
+        __syncthreads();
+        nnls_main(tikh, mu, nsetp, rnorm);
+    }
+
+    // ------------------------- Tikhonov cache (src/lsqnonneg.jl:392-444) -------------------------
+    // A hit needs exactly equal mu; re-solving the same mu reproduces the same bits, so the cache is a pure
+    // memo. Entries keep mu, rnorm, ||x||^2 and x (global scratch).
+    __device__ void cache_reset() {
+        if (L < kTikhCache) cmu[L] = NAN;
+        __syncthreads();
+    }
+    __device__ int cache_solve(double mu) {
+        int hit = -1, emp = -1;
+        for (int i = 0; i < kTikhCache; ++i) {
+            double c = rfl(cmu[i]);
+            if (hit < 0 && !isnan(c) && c == mu) hit = i;
+            if (emp < 0 && isnan(c)) emp = i;
+        }
+        if (hit >= 0) { ccur = hit; return hit; }
+        int slot = emp >= 0 ? emp : ((ccur + 1) % kTikhCache);
+        ccur = slot;
+        int ns;
+        double rn;
+        solve(A0, true, mu, ns, rn);
+        double v = (L < ns) ? x[idx[L]] : 0.0;
+        double semi = wsum(v * v);
+        if (L < n) cx[slot * n + L] = x[L];
+        if (L == 0) { cmu[slot] = mu; crn[slot] = rn; csemi[slot] = semi; }
+        __syncthreads();
+        return slot;
+    }
+    __device__ double cur_mu() { return rfl(cmu[ccur]); }
+    __device__ double cur_semi() { return rfl(csemi[ccur]); }
+    __device__ double cur_resnorm_sq() {
+        double mu = cur_mu(), rn = rfl(crn[ccur]);
+        return jl_max(rn * rn - (mu * mu) * cur_semi(), 0.0);
+    }
+    __device__ void take_cur_solution() {
+        if (L < n) xf[L] = cx[ccur * n + L];
+        __syncthreads();
+    }
+    __device__ double plain_solve_to_xf() {
+        int ns;
+        double rn;
+        solve(A0, false, 0.0, ns, rn);
+        if (L < n) xf[L] = x[L];
+        __syncthreads();
+        return rn;
+    }
+    __device__ double plain_solve_res2(int &ns) {
+        double rn;
+        solve(A0, false, 0.0, ns, rn);
+        return rn * rn;
+    }
+};
+
+// --------------------------------- scalar optimizers (src/optimization.jl) ---------------------------------
+// bracket_root_monotonic (:191-219)
+template <class F> __device__ void bracket_root_monotonic(F &&f, double a, double delta, double dilate, double mono,
+                                                          int maxiters, double out[4]) {
+    double fa = f(a);
+    if (!isfinite(fa)) { out[0] = a; out[1] = a; out[2] = NAN; out[3] = NAN; return; }
+    if (fa == 0) { out[0] = a; out[1] = a; out[2] = fa; out[3] = fa; return; }
+    double sgn = jl_sign(mono) * jl_sign(fa);
+    double b = a - sgn * delta, fb = f(b);
+    if (!isfinite(fb)) { out[0] = a; out[1] = a; out[2] = fa; out[3] = fa; return; }
+    if (fb == 0) { out[0] = b; out[1] = b; out[2] = fb; out[3] = fb; return; }
+    delta *= dilate;
+    int cnt = 0;
+    while (fa * fb > 0 && cnt < maxiters) {
+        a = b; fa = fb;
+        b = a - sgn * delta;
+        fb = f(b);
+        if (!isfinite(fb)) { out[0] = a; out[1] = a; out[2] = fa; out[3] = fa; return; }
+        if (fb == 0) { out[0] = b; out[1] = b; out[2] = fb; out[3] = fb; return; }
+        delta *= dilate;
+        cnt += 1;
+    }
+    if (a < b) { out[0] = a; out[1] = b; out[2] = fa; out[3] = fb; }
+    else { out[0] = b; out[1] = a; out[2] = fb; out[3] = fa; }
+}
+
+// brent_root (:71-128)
+template <class F> __device__ void brent_root(F &&f, double x0, double x1, double fx0, double fx1, double xatol,
+                                              double xrtol, double ftol, int maxiters, double &xo, double &fo) {
+    if (fx0 == 0) { xo = x0; fo = fx0; return; }
+    if (fx1 == 0) { xo = x1; fo = fx1; return; }
+    double a = x0, b = x1, fa = fx0, fb = fx1;
+    if (fabs(fa) < fabs(fb)) { double t = a; a = b; b = t; t = fa; fa = fb; fb = t; }
+    double c = x0, d = x0, fc = fx0;
+    bool mflag = true;
+    for (int iter = 1; iter <= maxiters; ++iter) {
+        if (fabs(b - a) <= 2 * (xatol + xrtol * fabs(b))) { xo = b; fo = fb; return; }
+        double s = 0.0;
+        s += a * fb * fc / (fa - fb) / (fa - fc);
+        s += b * fa * fc / (fb - fa) / (fb - fc);
+        s += c * fa * fb / (fc - fa) / (fc - fb);
+        if (isnan(s) || isinf(s)) s = a - fa * (b - a) / (fb - fa);
+        double u = (3 * a + b) / 4, v = b;
+        if (u > v) { double t = u; u = v; v = t; }
+        double tol = jl_max(xatol, xrtol * jl_max(jl_max(fabs(b), fabs(c)), fabs(d)));
+        if (!(u < s && s < v) || (mflag && fabs(s - b) >= fabs(b - c) / 2) || (!mflag && fabs(s - b) >= fabs(b - c) / 2) ||
+            (mflag && fabs(b - c) <= tol) || (!mflag && fabs(c - d) <= tol)) {
+            s = (a + b) / 2;
+            mflag = true;
+        } else {
+            mflag = false;
+        }
+        double fs = f(s);
+        if (fs == 0) { xo = s; fo = fs; return; }
+        if (isnan(fs) || isinf(fs)) { xo = b; fo = fb; return; }
+        if (fabs(fs) <= ftol) { xo = s; fo = fs; return; }
+        d = c; c = b; fc = fb;
+        if (jl_sign(fa) * jl_sign(fs) < 0) { b = s; fb = fs; }
+        else { a = s; fa = fs; }
+        if (fabs(fa) < fabs(fb)) { double t = a; a = b; b = t; t = fa; fa = fb; fb = t; }
+    }
+    xo = b; fo = fb;
+}
+
+// brent_minimize (:319-413)
+template <class F> __device__ void brent_minimize(F &&f, double x1, double x2, double xrtol, double xatol, int maxiters,
+                                                  double &xo, double &yo) {
+    const double alpha = 2 - kGolden;
+    double x = x1 + alpha * (x2 - x1);
+    double y = f(x);
+    double dx_old = 0.0, dx = 0.0, x_older = x, x_old = x, y_older = y, y_old = y;
+    int iter = 0;
+    while (iter < maxiters) {
+        double p = 0.0, q = 0.0, xm = (x2 + x1) / 2, dx_tol = xatol + xrtol * fabs(x);
+        if (fabs(x - xm) + (x2 - x1) / 2 <= 2 * dx_tol) break;
+        iter += 1;
+        if (fabs(dx_old) > dx_tol) {
+            double r = (x - x_old) * (y - y_older);
+            q = (x - x_older) * (y - y_old);
+            p = (x - x_older) * q - (x - x_old) * r;
+            q = 2 * (q - r);
+            if (q > 0) p = -p;
+            else q = -q;
+        }
+        if (fabs(p) < fabs(q * dx_old / 2) && p < q * (x2 - x) && p < q * (x - x1)) {
+            dx_old = dx;
+            dx = p / q;
+            double x_tmp = x + dx;
+            if ((x_tmp - x1) < 2 * dx_tol || (x2 - x_tmp) < 2 * dx_tol) dx = (x < xm) ? dx_tol : -dx_tol;
+        } else {
+            dx_old = (x < xm) ? x2 - x : x1 - x;
+            dx = alpha * dx_old;
+        }
+        double x_new = (fabs(dx) >= dx_tol) ? x + dx : x + ((dx > 0) ? dx_tol : -dx_tol);
+        double y_new = f(x_new);
+        if (y_new < y) {
+            if (x_new < x) x2 = x;
+            else x1 = x;
+            x_older = x_old; x_old = x; x = x_new;
+            y_older = y_old; y_old = y; y = y_new;
+        } else {
+            if (x_new < x) x1 = x_new;
+            else x2 = x_new;
+            if (y_new <= y_old || x_old == x) {
+                x_older = x_old; x_old = x_new;
+                y_older = y_old; y_old = y_new;
+            } else if (y_new <= y_older || x_older == x || x_older == x_old) {
+                x_older = x_new;
+                y_older = y_new;
+            }
+        }
+    }
+    xo = x;
+    yo = y;
+}
+
+// ------------------------------------ regularization (src/lsqnonneg.jl) ------------------------------------
+// lsqnonneg_chi2! with method :brent (:504-593)
+__device__ void reg_chi2(Wave &W, double chi2_target, double &mu_out, double &chi2_out) {
+    int ns;
+    double res2_min = W.plain_solve_res2(ns);
+    if (res2_min == 0 || ns == 0) {
+        if (W.L < W.n) W.xf[W.L] = W.x[W.L];
+        __syncthreads();
+        mu_out = 0.0; chi2_out = 1.0;
+        return;
+    }
+    if (W.L < W.n) W.xf[W.L] = W.x[W.L];  // x_unreg, kept for the no-root branch
+    __syncthreads();
+    const double target = chi2_target * res2_min;
+    W.cache_reset();
+    auto f = [&](double logmu) {
+        W.cache_solve(exp(logmu));
+        return (W.cur_resnorm_sq() - target) / target;
+    };
+    double br[4];
+    bracket_root_monotonic(f, -4.0, 1.0, 1.5, 1.0, 6, br);
+    double a = br[0], b = br[1], fa = br[2], fb = br[3], xf, ff;
+    if (fa * fb < 0) brent_root(f, a, b, fa, fb, 0.0, 0.0, 1e-3 * (chi2_target - 1), 100, xf, ff);
+    else if (!isfinite(fa)) { xf = b; ff = fb; }
+    else if (!isfinite(fb)) { xf = a; ff = fa; }
+    else if (fabs(fa) < fabs(fb)) { xf = a; ff = fa; }
+    else { xf = b; ff = fb; }
+    double mu_final, res2_final;
+    if (isfinite(ff)) {
+        mu_final = exp(xf);
+        res2_final = target * (1 + ff);
+        W.cache_solve(mu_final);
+        W.take_cur_solution();
+    } else {
+        mu_final = 0.0;
+        res2_final = 1.0;
+    }
+    mu_out = mu_final;
+    chi2_out = res2_final / res2_min;
+}
+
+// lsqnonneg_mdp! (:700-747)
+__device__ void reg_mdp(Wave &W, double delta, double &mu_out, double &chi2_out) {
+    int ns;
+    double res2_min = W.plain_solve_res2(ns);
+    if (W.L < W.n) W.xf[W.L] = W.x[W.L];
+    __syncthreads();
+    if (delta <= sqrt(res2_min)) { mu_out = 0.0; chi2_out = 1.0; return; }
+    double p = 0.0;
+    for (int i = W.L; i < W.m; i += kWave) p += W.b0[i] * W.b0[i];
+    double res2_max = wsum(p);
+    if (delta >= sqrt(res2_max)) {
+        if (W.L < W.n) W.xf[W.L] = 0.0;
+        __syncthreads();
+        mu_out = INFINITY;
+        chi2_out = res2_max / res2_min;
+        return;
+    }
+    W.cache_reset();
+    const double d2 = delta * delta;
+    auto f = [&](double logmu) {
+        W.cache_solve(exp(logmu));
+        return W.cur_resnorm_sq() - d2;
+    };
+    double br[4];
+    bracket_root_monotonic(f, -4.0, 1.0, 1.5, 1.0, 6, br);
+    double a = br[0], b = br[1], fa = br[2], fb = br[3], xf, ff;
+    if (fa * fb < 0) brent_root(f, a, b, fa, fb, 0.0, 0.0, 1e-3 * d2, 100, xf, ff);
+    else if (!isfinite(fa)) { xf = b; ff = fb; }
+    else if (!isfinite(fb)) { xf = a; ff = fa; }
+    else if (fabs(fa) < fabs(fb)) { xf = a; ff = fa; }
+    else { xf = b; ff = fb; }
+    double mu_final, res2_final;
+    if (isfinite(ff)) {
+        mu_final = exp(xf);
+        res2_final = d2 + ff;
+        W.cache_solve(mu_final);
+        W.take_cur_solution();
+    } else {
+        mu_final = 0.0;
+        res2_final = 1.0;
+    }
+    mu_out = mu_final;
+    chi2_out = res2_final / res2_min;
+}
+
+// Squared singular values of the m x n basis for gcv_dof (src/lsqnonneg.jl:1321-1329). The reference
+// calls LAPACK dgesdd (src/utils.jl:96-134); here: Gram matrix of the smaller side, Householder
+// tridiagonalization, then Sturm-sequence bisection, one eigenvalue per lane. Absolute accuracy is
+// eps * ||A||^2, far below the mu^2 >= exp(-16) scale at which gcv_dof uses them.
+__device__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
+    const int m = W.m, n = W.n, L = W.L, lda = W.lda;
+    if (L < n)
+        for (int i = 0; i < m; ++i) W.Aat(i, L) = W.A0[i * n + L];
+    __syncthreads();
+    N = min(m, n);
+    double *G = W.A + (size_t)m * lda;  // N x N at rows [m, m + N)
+    if (L < N) {
+        if (m >= n) {
+            for (int c = 0; c < N; ++c) {
+                double s = 0.0;
+                for (int i = 0; i < m; ++i) s = fma(W.Aat(i, L), W.Aat(i, c), s);
+                G[L * lda + c] = s;
+            }
+        } else {
+            for (int c = 0; c < N; ++c) {
+                double s = 0.0;
+                for (int j = 0; j < n; ++j) s = fma(W.Aat(L, j), W.Aat(c, j), s);
+                G[L * lda + c] = s;
+            }
+        }
+    }
+    W.flops += 2ull * N * N * (m >= n ? m : n);
+    __syncthreads();
+    double *dv = W.b, *ev = W.b + N, *vv = W.zz, *qv = W.zz + N;
+    for (int k = 0; k < N - 2; ++k) {
+        double xr = (L > k && L < N) ? G[L * lda + k] : 0.0;
+        double nrm = sqrt(wsum(xr * xr));
+        double x1 = rfl(G[(k + 1) * lda + k]);
+        if (nrm == 0.0) {
+            if (L == 0) ev[k] = x1;
+            __syncthreads();
+            continue;
+        }
+        double ah = -copysign(nrm, x1);
+        double vr = xr;
+        if (L == k + 1) vr = xr - ah;
+        double vn = sqrt(wsum(vr * vr));
+        if (vn == 0.0) {
+            if (L == 0) ev[k] = x1;
+            __syncthreads();
+            continue;
+        }
+        vr /= vn;
+        if (L < N) vv[L] = vr;
+        if (L == 0) ev[k] = ah;
+        __syncthreads();
+        double pr = 0.0;
+        if (L > k && L < N)
+            for (int c = k + 1; c < N; ++c) pr = fma(G[L * lda + c], vv[c], pr);
+        double Kq = wsum(vr * pr);
+        double qr = pr - Kq * vr;
+        if (L < N) qv[L] = qr;
+        __syncthreads();
+        if (L > k && L < N)
+            for (int c = k + 1; c < N; ++c) G[L * lda + c] -= 2.0 * (vr * qv[c] + qr * vv[c]);
+        W.flops += 8ull * (N - k) * (N - k);
+        __syncthreads();
+    }
+    if (L < N) dv[L] = G[L * lda + L];
+    if (N >= 2 && L == 0) ev[N - 2] = G[(N - 1) * lda + (N - 2)];
+    __syncthreads();
+    // Gershgorin interval
+    double lo = INFINITY, hi = -INFINITY;
+    if (L < N) {
+        double r = 0.0;
+        if (L > 0) r += fabs(ev[L - 1]);
+        if (L < N - 1) r += fabs(ev[L]);
+        lo = dv[L] - r;
+        hi = dv[L] + r;
+    }
+    lo = wmind(lo);
+    hi = wmaxd(hi);
+    double gnorm = fmax(fabs(lo), fabs(hi));
+    double a = lo, b = hi;
+    const int t = L;
+    int it = 0;
+    for (; it < 200; ++it) {
+        double mid = 0.5 * (a + b);
+        bool done = !(t < N) || (b - a) <= fmax(2.0 * kEps * fmax(fabs(a), fabs(b)), kEps * gnorm) || mid <= a || mid >= b;
+        if (ballot(!done) == 0) break;
+        if (!done) {
+            int cnt = 0;
+            double q = dv[0] - mid;
+            cnt += q < 0;
+            for (int i = 1; i < N; ++i) {
+                if (q == 0.0) q = kEps * (fabs(ev[i - 1]) + kEps);
+                q = (dv[i] - mid) - (ev[i - 1] * ev[i - 1]) / q;
+                cnt += q < 0;
+            }
+            if (cnt > t) b = mid;
+            else a = mid;
+        }
+    }
+    W.flops += 5ull * N * N * it;
+    lam_lane = (t < N) ? fmax(0.5 * (a + b), 0.0) : 0.0;
+    __syncthreads();
+}
+
+// lsqnonneg_gcv! with method :brent (:1136-1205)
+__device__ void reg_gcv(Wave &W, double &mu_out, double &chi2_out) {
+    double lam;
+    int N;
+    gram_eigs(W, lam, N);
+    const int m = W.m, n = W.n;
+    const double low = (kEps * kEps) / m;
+    W.cache_reset();
+    auto f = [&](double logmu) {
+        double mu = exp(logmu);
+        W.cache_solve(mu);
+        double res2 = W.cur_resnorm_sq();
+        double l2 = mu * mu;
+        double term = (W.L < N) ? l2 / (lam + l2) : 0.0;
+        double dof = (double)(m - n > 0 ? m - n : 0) + wsum(term);
+        double g = res2 / (dof * dof);
+        return log(jl_max(g, low));
+    };
+    double xf, yf;
+    brent_minimize(f, -8.0, 2.0, 0.0, 1e-4, 20, xf, yf);
+    double mu_final = exp(xf);
+    W.cache_solve(mu_final);
+    W.take_cur_solution();
+    double res2 = W.cur_resnorm_sq();
+    int ns;
+    double res2u = W.plain_solve_res2(ns);
+    mu_out = mu_final;
+    chi2_out = res2 / res2u;
+}
+
+// lsqnonneg_lcurve! (:812-840) with lcurve_corner (:872-972) and the isapprox-keyed GrowableCache /
+// CachedFunction / mapfindmax semantics of src/utils.jl:140-254. Caches live in global scratch; scans are
+// lane-parallel reductions that reproduce the reference's sequential first-match / first-max rules.
+struct LCurve {
+    Wave &W;
+    double *fcx, *fcP0, *fcP1;
+    double *pcx, *pcP0, *pcP1, *pcC;
+    double *sx[4], *sP[4][2];
+    int nfc, npc, nsc;
+    bool overflow;
+
+    __device__ LCurve(Wave &w) : W(w), nfc(0), npc(0), nsc(0), overflow(false) {
+        double *p = w.lc;
+        fcx = p; fcP0 = p + kLcurveCap; fcP1 = p + 2 * kLcurveCap;
+        p += 3 * kLcurveCap;
+        pcx = p; pcP0 = p + kLcurveCap; pcP1 = p + 2 * kLcurveCap; pcC = p + 3 * kLcurveCap;
+        p += 4 * kLcurveCap;
+        for (int i = 0; i < 4; ++i) { sx[i] = p; p += kLcurveCap; }
+        for (int i = 0; i < 4; ++i)
+            for (int c = 0; c < 2; ++c) { sP[i][c] = p; p += kLcurveCap; }
+    }
+    __device__ int find(const double *keys, int cnt, double x) {
+        int first = INT_MAX;
+        for (int base = 0; base < cnt; base += kWave) {
+            int k = base + W.L;
+            bool hit = k < cnt && jl_isapprox(x, keys[k]);
+            int f = first_set(ballot(hit));
+            if (f >= 0) { first = base + f; break; }
+        }
+        return first == INT_MAX ? -1 : first;
+    }
+    __device__ void eval(double x, double &P0, double &P1) {
+        int i = find(fcx, nfc, x);
+        if (i >= 0) { P0 = rfl(fcP0[i]); P1 = rfl(fcP1[i]); return; }
+        W.cache_solve(exp(x));
+        P0 = log(W.cur_resnorm_sq());
+        P1 = log(W.cur_semi());
+        if (nfc >= kLcurveCap) { overflow = true; return; }
+        if (W.L == 0) { fcx[nfc] = x; fcP0[nfc] = P0; fcP1[nfc] = P1; }
+        nfc++;
+        __syncthreads();
+    }
+    __device__ void pc_set(double x, double P0, double P1, double C) {
+        int i = find(pcx, npc, x);
+        if (i < 0) {
+            if (npc >= kLcurveCap) { overflow = true; return; }
+            i = npc++;
+            if (W.L == 0) pcx[i] = x;
+        }
+        if (W.L == 0) { pcP0[i] = P0; pcP1[i] = P1; pcC[i] = C; }
+        __syncthreads();
+    }
+    __device__ int pc_argmax() {
+        // findmax with isless (NaN largest), first occurrence
+        double bc = -INFINITY;
+        int bi = INT_MAX;
+        bool bnan = false;
+        for (int base = 0; base < npc; base += kWave) {
+            int k = base + W.L;
+            if (k < npc) {
+                double c = pcC[k];
+                bool cn = isnan(c);
+                bool better = (bi == INT_MAX) || (cn && !bnan) || (!cn && !bnan && c > bc);
+                if (better) { bc = c; bi = k; bnan = cn; }
+            }
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            double oc = __shfl_xor(bc, o, kWave);
+            int oi = __shfl_xor(bi, o, kWave);
+            bool on = __shfl_xor((int)bnan, o, kWave) != 0;
+            bool take;
+            if (oi == INT_MAX) take = false;
+            else if (bi == INT_MAX) take = true;
+            else if (on != bnan) take = on;
+            else if (on) take = oi < bi;
+            else take = jl_isless(bc, oc) || (oc == bc && oi < bi);
+            if (take) { bc = oc; bi = oi; bnan = on; }
+        }
+        return rfl(bi);
+    }
+    __device__ void update_curvature(const double xs[4], const double Ps[4][2], const double Ptl[2], const double Pbr[2],
+                                     double Ctol) {
+        for (int i = 0; i < 4; ++i) {
+            double x = xs[i], C = -INFINITY;
+            const double *P = Ps[i];
+            double d1 = norm2(P[0] - Ptl[0], P[1] - Ptl[1]), d2 = norm2(P[0] - Pbr[0], P[1] - Pbr[1]);
+            if (jl_min(d1, d2) > Ctol) {
+                // x- = largest key < x (first such entry), x+ = smallest key > x (first such entry)
+                double lv = -INFINITY, rv = INFINITY;
+                int li = INT_MAX, ri = INT_MAX;
+                for (int base = 0; base < npc; base += kWave) {
+                    int k = base + W.L;
+                    if (k < npc) {
+                        double xk = pcx[k];
+                        if (xk < x && (li == INT_MAX || xk > lv)) { lv = xk; li = k; }
+                        if (xk > x && (ri == INT_MAX || xk < rv)) { rv = xk; ri = k; }
+                    }
+                }
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) {
+                    double olv = __shfl_xor(lv, o, kWave), orv = __shfl_xor(rv, o, kWave);
+                    int oli = __shfl_xor(li, o, kWave), ori = __shfl_xor(ri, o, kWave);
+                    if (oli != INT_MAX && (li == INT_MAX || olv > lv || (olv == lv && oli < li))) { lv = olv; li = oli; }
+                    if (ori != INT_MAX && (ri == INT_MAX || orv < rv || (orv == rv && ori < ri))) { rv = orv; ri = ori; }
+                }
+                li = rfl(li);
+                ri = rfl(ri);
+                double Pm0 = P[0], Pm1 = P[1], Pp0 = P[0], Pp1 = P[1];
+                if (li != INT_MAX) { Pm0 = rfl(pcP0[li]); Pm1 = rfl(pcP1[li]); }
+                if (ri != INT_MAX) { Pp0 = rfl(pcP0[ri]); Pp1 = rfl(pcP1[ri]); }
+                // menger (:967-972)
+                double djk0 = Pm0 - P[0], djk1 = Pm1 - P[1];
+                double dkl0 = P[0] - Pp0, dkl1 = P[1] - Pp1;
+                double dlj0 = Pp0 - Pm0, dlj1 = Pp1 - Pm1;
+                double a = djk0 * djk0 + djk1 * djk1, b = dkl0 * dkl0 + dkl1 * dkl1, c = dlj0 * dlj0 + dlj1 * dlj1;
+                C = 2 * (djk0 * dkl1 - djk1 * dkl0) / sqrt(a * b * c);
+            }
+            pc_set(x, P[0], P[1], C);
+        }
+    }
+    __device__ double corner(double xlow, double xhigh, double xtol, double Ptol, double Ctol) {
+        double xs[4], Ps[4][2];
+        xs[0] = xlow;
+        xs[3] = xhigh;
+        xs[1] = (kGolden * xs[0] + xs[3]) / (kGolden + 1);
+        xs[2] = xs[0] + (xs[3] - xs[1]);
+        for (int i = 0; i < 4; ++i) eval(xs[i], Ps[i][0], Ps[i][1]);
+        for (int i = 0; i < 4; ++i) {
+            if (npc >= kLcurveCap) { overflow = true; break; }
+            if (W.L == 0) { pcx[npc] = xs[i]; pcP0[npc] = Ps[i][0]; pcP1[npc] = Ps[i][1]; pcC[npc] = -INFINITY; }
+            npc++;
+        }
+        __syncthreads();
+        const double Ptl[2] = {Ps[0][0], Ps[0][1]}, Pbr[2] = {Ps[3][0], Ps[3][1]};
+        update_curvature(xs, Ps, Ptl, Pbr, Ctol);
+        int iter = 0;
+        while (!overflow && !(fabs(xs[3] - xs[0]) < xtol || norm2(Ps[0][0] - Ps[3][0], Ps[0][1] - Ps[3][1]) < Ptol)) {
+            iter += 1;
+            // backtracking: the last state whose x2 or x3 equals the best point and whose width is the
+            // smallest width not exceeding the current one (same state the sequential scan ends on)
+            double xb = rfl(pcx[pc_argmax()]);
+            double w0 = fabs(xs[3] - xs[0]);
+            double bw = INFINITY;
+            int bk = -1;
+            for (int base = 0; base < nsc; base += kWave) {
+                int k = base + W.L;
+                if (k < nsc) {
+                    double s1 = sx[1][k], s2 = sx[2][k], wk = fabs(sx[3][k] - sx[0][k]);
+                    if ((s1 == xb || s2 == xb) && wk <= w0 && (bk < 0 || wk < bw || (wk == bw && k > bk))) { bw = wk; bk = k; }
+                }
+            }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                double ow = __shfl_xor(bw, o, kWave);
+                int ok = __shfl_xor(bk, o, kWave);
+                if (ok >= 0 && (bk < 0 || ow < bw || (ow == bw && ok > bk))) { bw = ow; bk = ok; }
+            }
+            bk = rfl(bk);
+            if (bk >= 0) {
+                for (int i = 0; i < 4; ++i) {
+                    xs[i] = rfl(sx[i][bk]);
+                    Ps[i][0] = rfl(sP[i][0][bk]);
+                    Ps[i][1] = rfl(sP[i][1][bk]);
+                }
+            }
+            int i2 = find(pcx, npc, xs[1]), i3 = find(pcx, npc, xs[2]);
+            double C2 = rfl(pcC[i2]), C3 = rfl(pcC[i3]);
+            if (C2 > C3) {
+                // move_left (:933-939)
+                double nx[4] = {xs[0], (kGolden * xs[0] + xs[2]) / (kGolden + 1), xs[1], xs[2]};
+                double nP[4][2];
+                nP[0][0] = Ps[0][0]; nP[0][1] = Ps[0][1];
+                eval(nx[1], nP[1][0], nP[1][1]);
+                nP[2][0] = Ps[1][0]; nP[2][1] = Ps[1][1];
+                nP[3][0] = Ps[2][0]; nP[3][1] = Ps[2][1];
+                for (int i = 0; i < 4; ++i) { xs[i] = nx[i]; Ps[i][0] = nP[i][0]; Ps[i][1] = nP[i][1]; }
+            } else {
+                // move_right (:941-946)
+                double nx[4] = {xs[1], xs[2], xs[1] + (xs[3] - xs[2]), xs[3]};
+                double nP[4][2];
+                nP[0][0] = Ps[1][0]; nP[0][1] = Ps[1][1];
+                nP[1][0] = Ps[2][0]; nP[1][1] = Ps[2][1];
+                eval(nx[2], nP[2][0], nP[2][1]);
+                nP[3][0] = Ps[3][0]; nP[3][1] = Ps[3][1];
+                for (int i = 0; i < 4; ++i) { xs[i] = nx[i]; Ps[i][0] = nP[i][0]; Ps[i][1] = nP[i][1]; }
+            }
+            update_curvature(xs, Ps, Ptl, Pbr, Ctol);
+            if (nsc >= kLcurveCap) { overflow = true; break; }
+            if (W.L == 0) {
+                for (int i = 0; i < 4; ++i) { sx[i][nsc] = xs[i]; sP[i][0][nsc] = Ps[i][0]; sP[i][1][nsc] = Ps[i][1]; }
+            }
+            nsc++;
+            __syncthreads();
+        }
+        return rfl(pcx[pc_argmax()]);
+    }
+};
+
+__device__ void reg_lcurve(Wave &W, double &mu_out, double &chi2_out) {
+    W.cache_reset();
+    LCurve lc(W);
+    double logmu = lc.corner(-8.0, 2.0, 1e-4, 1e-4, 1e-4);
+    if (lc.overflow) W.status |= kStatusLcurveOverflow;
+    double mu_final = exp(logmu);
+    W.cache_solve(mu_final);
+    W.take_cur_solution();
+    double res2 = W.cur_resnorm_sq();
+    int ns;
+    double res2u = W.plain_solve_res2(ns);
+    mu_out = mu_final;
+    chi2_out = res2 / res2u;
+}
+
+// ------------------------------------ flip-angle search ------------------------------------
+// optimize_flip_angle! (src/T2mapSEcorr.jl:409-423): cubic Hermite surrogate (src/splines.jl:504-566)
+// driven by the discrete bisection searcher (:705-850) over the angle grid; each evaluation is an NNLS
+// solve on the grid basis plus the surrogate gradient (:1010-1041). Lane k owns grid angle k.
+__device__ __forceinline__ double evalpoly4(double t, double c0, double c1, double c2, double c3) {
+    return fma(t, fma(t, fma(t, c3, c2), c1), c0);
+}
+// CubicHermiteInterpolator + minimize (src/splines.jl:62-110)
+__device__ void hermite_minimize(double a, double b, double u0, double u1, double m0, double m1, double &xo, double &uo) {
+    double r = (b - a) / 2;
+    m0 = r * m0;
+    m1 = r * m1;
+    double du = u1 - u0, dm = m1 - m0, su = u1 + u0, sm = m1 + m0;
+    double c0 = su / 2 - dm / 4, c1 = (3 * du - sm) / 4, c2 = dm / 4, c3 = (sm - du) / 4;
+    double xend = u0 < u1 ? a : b, uend = u0 < u1 ? u0 : u1;
+    double D = 3 * (u0 - u1), th = D / 2 + (m0 + m1), g = th * th - m0 * m1;
+    g = g > 0 ? -sqrt(g) : 0.0;
+    double p = -(D + (m0 + m1)), q = 2 * g + (m0 - m1);
+    if (fabs(p) < fabs(q)) {
+        double t = p / q, y = evalpoly4(t, c0, c1, c2, c3);
+        if (y < uend) {
+            double c = (a + b) / 2, rr = (b - a) / 2;
+            double x = fma(rr, t, c);
+            xo = x < a ? a : (x > b ? b : x);
+            uo = y;
+            return;
+        }
+    }
+    xo = xend;
+    uo = uend;
+}
+
+struct FlipSearch {
+    Wave &W;
+    const double *As, *dAs;
+    int K, numeval, mineval;
+    unsigned long long seen;
+    double al, u, du;  // lane k: grid angle, loss, gradient
+
+    __device__ void eval(int k) {
+        const int m = W.m, n = W.n;
+        const double *A = As + (size_t)k * m * n, *dA = dAs + (size_t)k * m * n;
+        int ns;
+        double rn;
+        W.solve(A, false, 0.0, ns, rn);
+        double ell = rn * rn;
+        double part = 0.0;
+        for (int i = W.L; i < m; i += kWave) {
+            double ax = 0.0, dax = 0.0;
+            for (int j = 0; j < n; ++j) {
+                double xj = W.x[j];
+                if (xj > 0) { ax = fma(xj, A[i * n + j], ax); dax = fma(xj, dA[i * n + j], dax); }
+            }
+            ax -= W.b0[i];
+            part += dax * ax;
+        }
+        double dt = wsum(part);
+        W.flops += 4ull * m * n + 2ull * m;
+        if (W.L == k) { u = ell; du = 2 * dt; }
+    }
+    // update!(surr, state, I; maxeval) (:736-744)
+    __device__ bool update(int k) {
+        if (numeval >= K) return true;
+        if ((seen >> k) & 1ull) return false;
+        eval(k);
+        seen |= 1ull << k;
+        numeval += 1;
+        return numeval >= K;
+    }
+    __device__ bool box_evaluated(int lo, int hi) { return ((seen >> lo) & 1ull) && ((seen >> hi) & 1ull); }
+    // evaluate_box! (:802-815); natural corner order, or sorted by distance to x
+    __device__ void evaluate_box(int lo, int hi, bool sorted, double x) {
+        int c0 = lo, c1 = hi;
+        if (sorted) {
+            double alo = readlane(al, lo), ahi = readlane(al, hi);
+            double d0 = (alo - x) * (alo - x), d1 = (ahi - x) * (ahi - x);
+            if (d1 < d0) { c0 = hi; c1 = lo; }
+        }
+        if (box_evaluated(lo, hi)) return;
+        if (update(c0)) return;
+        if (box_evaluated(lo, hi)) return;
+        update(c1);
+    }
+    // suggest_point (:544-566): one Hermite segment per evaluated lane, earliest minimum wins.
+    __device__ void suggest(double &xo, double &uo) {
+        const int k = W.L;
+        int f = first_set(seen);
+        double uf = readlane(u, f), xf = readlane(al, f);
+        unsigned long long below = (k == 0) ? 0ull : (seen & ((1ull << k) - 1ull));
+        int p = last_set(below);
+        int src = p >= 0 ? p : k;
+        double ap = __shfl(al, src, kWave), up = __shfl(u, src, kWave), dp = __shfl(du, src, kWave);
+        double ys = INFINITY;
+        int ord = INT_MAX;
+        double xs = 0.0;
+        if (((seen >> k) & 1ull) && p >= 0) {
+            double xx, yy;
+            hermite_minimize(ap, al, up, u, dp, du, xx, yy);
+            if (yy < uf) { ys = yy; ord = k; xs = xx; }
+        }
+        W.flops += 30ull;
+        wargmin(ys, ord);
+        if (isnan(uf) || ord == INT_MAX) { xo = xf; uo = uf; return; }
+        xo = readlane(xs, ord);
+        uo = ys;
+    }
+    __device__ double run() {
+        seen = 0ull;
+        numeval = 0;
+        // initialize! (:716-734): depth-limited pre-order bisection, explicit stack
+        for (int depth = 1; depth <= mineval; ++depth) {
+            int sp = 0;
+            if (W.L == 0) { W.stk[0] = 0; W.stk[1] = K - 1; W.stk[2] = depth; }
+            sp = 1;
+            __syncthreads();
+            while (sp > 0) {
+                sp -= 1;
+                int lo = rfl(W.stk[3 * sp]), hi = rfl(W.stk[3 * sp + 1]), dp = rfl(W.stk[3 * sp + 2]);
+                __syncthreads();
+                if (dp <= 0) continue;
+                evaluate_box(lo, hi, false, 0.0);
+                if (numeval >= mineval) continue;
+                int mid = (lo + hi) / 2;
+                if (W.L == 0) {
+                    W.stk[3 * sp] = mid; W.stk[3 * sp + 1] = hi; W.stk[3 * sp + 2] = dp - 1;
+                    W.stk[3 * sp + 3] = lo; W.stk[3 * sp + 4] = mid; W.stk[3 * sp + 5] = dp - 1;
+                }
+                sp += 2;
+                __syncthreads();
+            }
+            if (numeval >= mineval) break;
+        }
+        // bisection_search (:750-775) with minimal_bounding_box (:778-800)
+        double x, uu;
+        suggest(x, uu);
+        for (;;) {
+            int lo = 0, hi = K - 1, blo, bhi;
+            for (;;) {
+                int mid = (lo + hi) / 2;
+                double alo = readlane(al, lo), amid = readlane(al, mid);
+                if (alo <= x && x <= amid) {
+                    if (!box_evaluated(lo, mid) || !(mid - lo > 1)) { blo = lo; bhi = mid; break; }
+                    hi = mid;
+                } else {
+                    if (!box_evaluated(mid, hi) || !(hi - mid > 1)) { blo = mid; bhi = hi; break; }
+                    lo = mid;
+                }
+            }
+            evaluate_box(blo, bhi, true, x);
+            suggest(x, uu);
+            if (numeval >= K || (bhi - blo) <= 1) break;
+        }
+        return x;
+    }
+};
+
+// Basis at one flip angle: n EPG curves, two per wave (32-lane groups) when the echo window fits.
+__device__ void voxel_basis(Wave &W, bool general, double alpha, double beta, double E1, const double *E2,
+                            double *A0out) {
+    const int m = W.m, n = W.n, L = W.L;
+    if (m / 2 + 2 <= 32) {
+        const int h = L >> 5, s = L & 31;
+        for (int jp = 0; jp < n; jp += 2) {
+            int j = jp + h;
+            bool valid = j < n;
+            int jj = valid ? j : n - 1;
+            epg_curve<32, double>(general, alpha, beta, E1, E2[jj], m, s, valid, A0out + jj, nullptr, n);
+        }
+    } else {
+        for (int j = 0; j < n; ++j) epg_curve<64, double>(general, alpha, beta, E1, E2[j], m, L, true, A0out + j, nullptr, n);
+    }
+    // EPG op count of the reference's split recursion: ~13 flops per retained state per echo
+    W.flops += (unsigned long long)n * 13ull * ((unsigned long long)m * m / 4 + m);
+    __syncthreads();
+}
+
+// ------------------------------------ the voxel kernel ------------------------------------
+template <int REG>
+__global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
+    extern __shared__ double smem[];
+    Wave W;
+    W.L = lane_id();
+    W.m = p.m;
+    W.n = p.n;
+    W.lda = p.lda;
+    W.mw = p.mw;
+    double *s = smem;
+    W.A = s; s += (size_t)p.mw * p.lda;
+    W.b = s; s += p.mw;
+    W.zz = s; s += p.mw;
+    W.x = s; s += kMaxT2;
+    W.w = s; s += kMaxT2;
+    W.xf = s; s += kMaxT2;
+    W.b0 = s; s += p.m;
+    W.cmu = s; s += kTikhCache;
+    W.crn = s; s += kTikhCache;
+    W.csemi = s; s += kTikhCache;
+    int *si = reinterpret_cast<int *>(s);
+    W.idx = si; si += kMaxT2;
+    W.diag = si; si += kMaxT2;
+    W.stk = si;
+    double *slot = p.scratch + (int64_t)blockIdx.x * p.slot_stride;
+    W.A0s = slot;
+    W.cx = slot + (int64_t)p.m * p.n;
+    W.lc = W.cx + (int64_t)kTikhCache * p.n;
+    W.flops = 0;
+    W.solves = 0;
+    W.status = 0;
+    W.ccur = 0;
+    const int m = p.m, n = p.n, L = W.L;
+    const int64_t V = p.V;
+    for (;;) {
+        unsigned int vi = 0;
+        if (L == 0) vi = atomicAdd(p.counter, 1u);
+        vi = (unsigned)rfl((int)__shfl((int)vi, 0, kWave));
+        if ((int64_t)vi >= p.nvox) break;
+        const int64_t I = p.vox[vi];
+        // voxelwise_T2_distribution! (:201-238): normalize by the max echo
+        double mx = 0.0;
+        for (int i = L; i < m; i += kWave) {
+            double bi = p.image[I + V * i];
+            mx = bi > mx ? bi : mx;
+            W.b0[i] = bi;
+        }
+        double max_signal = wmaxd(mx);
+        __syncthreads();
+        if (max_signal > 0)
+            for (int i = L; i < m; i += kWave) W.b0[i] /= max_signal;
+        __syncthreads();
+        double alpha_v;
+        if (p.mode == kModeFixed) {
+            alpha_v = p.setflip;
+            W.A0 = p.Afixed;
+        } else {
+            if (p.mode == kModeB1) {
+                alpha_v = p.alpha_in[I];
+            } else {
+                FlipSearch fs{W, p.As, p.dAs, p.K, 0, p.mineval, 0ull, 0.0, 0.0, 0.0};
+                fs.al = (L < p.K) ? p.alphas[L] : p.alphas[p.K - 1];
+                fs.u = NAN;
+                fs.du = NAN;
+                alpha_v = fs.run();
+            }
+            voxel_basis(W, p.general != 0, alpha_v, p.beta, p.E1, p.E2, W.A0s);
+            W.A0 = W.A0s;
+        }
+        // T2_distribution! (src/T2mapSEcorr.jl:475-505)
+        double muv = 0.0, chi2v = 1.0;
+        if constexpr (REG == 0) W.plain_solve_to_xf();
+        else if constexpr (REG == 1) reg_lcurve(W, muv, chi2v);
+        else if constexpr (REG == 2) reg_gcv(W, muv, chi2v);
+        else if constexpr (REG == 3) reg_chi2(W, p.chi2factor, muv, chi2v);
+        else {
+            double sigma = p.noiselevel / max_signal;
+            double delta = sqrt((double)m) * sigma;
+            reg_mdp(W, delta, muv, chi2v);
+        }
+        // save_results! (:512-591)
+        double xs = (L < n) ? W.xf[L] * max_signal : 0.0;
+        if (L < n) W.xf[L] = xs;
+        __syncthreads();
+        // residuals of rows L and L + 64 (m <= 124 < 128)
+        double res0 = 0.0, res1 = 0.0;
+        double pr2 = 0.0, prs = 0.0;
+        for (int i = L; i < m; i += kWave) {
+            double fit = 0.0;
+            for (int j = 0; j < n; ++j) fit += W.A0[i * n + j] * W.xf[j];
+            double bi = W.b0[i] * max_signal;
+            double res = fit - bi;
+            if (i == L) res0 = res; else res1 = res;
+            pr2 += res * res;
+            prs += res;
+            if (p.decaycurve) p.decaycurve[I + V * i] = fit;
+        }
+        double sum_dist = wsum(xs);
+        double sum_res2 = wsum(pr2);
+        double mean_res = wsum(prs) / m;
+        double pv = 0.0;
+        if (L < m) pv += (res0 - mean_res) * (res0 - mean_res);
+        if (L + kWave < m) pv += (res1 - mean_res) * (res1 - mean_res);
+        double sd_res = sqrt(wsum(pv) / (m - 1));
+        double lt = (L < n) ? p.logT2[L] : 0.0;
+        double log_ggm = wsum(xs * lt) / sum_dist;
+        double dv = (L < n) ? (lt - log_ggm) * (lt - log_ggm) * xs : 0.0;
+        double log1p_gva = wsum(dv) / sum_dist;
+        W.flops += 2ull * m * n + 12ull * m + 8ull * n;
+        if (L == 0) {
+            p.gdn[I] = sum_dist;
+            p.ggm[I] = exp(log_ggm);
+            p.gva[I] = expm1(log1p_gva);
+            p.fnr[I] = sum_dist / sqrt(sum_res2 / (m - 1));
+            p.snr[I] = max_signal / sd_res;
+            p.alpha[I] = alpha_v;
+            if (p.mu) p.mu[I] = muv;
+            if (p.chi2f) p.chi2f[I] = chi2v;
+            if (p.resnorm) p.resnorm[I] = sqrt(sum_res2);
+        }
+        if (L < n) p.dist[I + V * L] = xs;
+        if (p.decaybasis && p.mode != kModeFixed && L < n)
+            for (int i = 0; i < m; ++i) p.decaybasis[I + V * (i + (int64_t)m * L)] = W.A0[i * n + L];
+        __syncthreads();
+    }
+    if (L == 0) {
+        atomicAdd(&p.counters[0], W.flops);
+        atomicAdd(&p.counters[1], W.solves);
+        if (W.status) atomicOr(p.status, W.status);
+    }
+}
+
+// ------------------------------------ T2partSEcorr ------------------------------------
+// voxelwise_T2_parts! (src/T2partSEcorr.jl:95-138): one thread per voxel, coalesced over voxels.
+__global__ void t2part_kernel(T2partParams p) {
+    for (int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I < p.V; I += (int64_t)gridDim.x * blockDim.x) {
+        bool bad = false;
+        for (int j = 0; j < p.n; ++j) bad |= isnan(p.dist[I + p.V * j]);
+        if (bad) continue;
+        double S = 0, Ssp = 0, Smp = 0, dsp = 0, dmp = 0, dw = 0;
+        for (int j = 0; j < p.n; ++j) {
+            double d = p.dist[I + p.V * j];
+            S += d;
+            if (p.weights) dw += d * p.weights[j];
+        }
+        for (int j = p.sp0; j <= p.sp1; ++j) { double d = p.dist[I + p.V * j]; dsp += d * p.logT2[j]; Ssp += d; }
+        for (int j = p.mp0; j <= p.mp1; ++j) { double d = p.dist[I + p.V * j]; dmp += d * p.logT2[j]; Smp += d; }
+        if (S > 0) {
+            p.sfr[I] = p.weights ? dw / S : Ssp / S;
+            p.mfr[I] = Smp / S;
+        }
+        if (Ssp > 0) p.sgm[I] = exp(dsp / Ssp);
+        if (Smp > 0) p.mgm[I] = exp(dmp / Smp);
+    }
+}
+
+}  // namespace
+
+// ------------------------------------ host-side launchers ------------------------------------
+size_t t2map_lds_bytes(int m, int n, int lda, int mw) {
+    size_t d = (size_t)mw * lda + 2 * (size_t)mw + 3 * kMaxT2 + (size_t)m + 3 * kTikhCache;
+    size_t i = 2 * kMaxT2 + 3 * (2 * kMaxAngles + 2);
+    return d * sizeof(double) + i * sizeof(int);
+}
+
+int64_t t2map_slot_doubles(int m, int n) {
+    int64_t s = (int64_t)m * n + (int64_t)kTikhCache * n + 19 * (int64_t)kLcurveCap;
+    return (s + 31) / 32 * 32;
+}
+
+hipError_t launch_grid_basis(const GridParams &p, hipStream_t s) {
+    if (p.m / 2 + 2 <= 32) {
+        dim3 grid(p.K, (p.n + 1) / 2);
+        hipLaunchKernelGGL(grid_basis_kernel<32>, grid, dim3(64), 0, s, p);
+    } else {
+        dim3 grid(p.K, p.n);
+        hipLaunchKernelGGL(grid_basis_kernel<64>, grid, dim3(64), 0, s, p);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_select(const double *image, int64_t V, double thr, int64_t *vox, unsigned int *count, hipStream_t s) {
+    int blocks = (int)std::min<int64_t>((V + 255) / 256, 4096);
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(select_kernel, dim3(blocks), dim3(256), 0, s, image, V, thr, vox, count);
+    return hipGetLastError();
+}
+
+using T2mapKernelFn = void (*)(T2mapParams);
+static T2mapKernelFn t2map_kernel_for(int reg) {
+    switch (reg) {
+    case 0: return t2map_kernel<0>;
+    case 1: return t2map_kernel<1>;
+    case 2: return t2map_kernel<2>;
+    case 3: return t2map_kernel<3>;
+    default: return t2map_kernel<4>;
+    }
+}
+
+hipError_t launch_t2map(const T2mapParams &p, int grid, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(t2map_kernel_for(p.reg), dim3(grid), dim3(64), lds, s, p);
+    return hipGetLastError();
+}
+
+hipError_t t2map_occupancy(int reg, int *blocks_per_cu, size_t lds) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, t2map_kernel_for(reg), 64, lds);
+}
+
+hipError_t launch_t2part(const T2partParams &p, hipStream_t s) {
+    int blocks = (int)std::min<int64_t>((p.V + 255) / 256, 8192);
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(t2part_kernel, dim3(blocks), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+}  // namespace dmi

@@ -1,0 +1,55 @@
+"""Builds libdecaes_amd.so (gfx950 kernels + C ABI) in-tree with hipcc."""
+from __future__ import annotations
+
+import os
+import subprocess
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent            # decaes.jl_amd/
+REPO = ROOT.parent
+CSRC = ROOT / "csrc"
+LIB = PKG / "libdecaes_amd.so"
+
+SOURCES = ["dmi_kernels.hip", "dmi_api.cpp", "dmi_synth.cpp"]
+
+HIPCC_FLAGS = [
+    "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
+    # fma only where the reference writes muladd; never contract elsewhere
+    "-ffp-contract=off",
+    # no packed FP32 instructions (v_pk_*_f32) anywhere in device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def hipcc() -> str:
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    return str(Path(rocm) / "bin" / "hipcc")
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile every source into decaes_amd/libdecaes_amd.so (always recompiles)."""
+    objs = []
+    obj_dir = ROOT / "_build"
+    obj_dir.mkdir(exist_ok=True)
+    for src in SOURCES:
+        obj = obj_dir / (src + ".o")
+        cmd = [hipcc(), *HIPCC_FLAGS, "-c", "-I", str(REPO / "include"), "-I", str(CSRC), str(CSRC / src), "-o", str(obj)]
+        if src.endswith(".cpp"):
+            cmd = [c for c in cmd if c not in ("--offload-arch=gfx950",)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(str(obj))
+    tmp = LIB.with_suffix(".so.tmp")
+    link = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp), "-lpthread"]
+    if verbose:
+        print(" ".join(link), flush=True)
+    subprocess.run(link, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

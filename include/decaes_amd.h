@@ -1,0 +1,132 @@
+/*
+ * decaes_amd.h -- C ABI of the MI355X (gfx950) T2mapSEcorr / T2partSEcorr path.
+ *
+ * Drop-in boundary for DECAES.jl v0.6.1's voxelwise path. The reference has no FFI of its own: its boundary
+ * is the Julia API T2mapSEcorr!/T2partSEcorr, so each entry point below names the reference function it
+ * replaces (file:line in the DECAES.jl tree). All arrays are Julia column-major; the library never keeps a
+ * caller pointer after a call returns. Every function returns DMI_OK (0) or a negative DMI_E* code;
+ * dmi_last_error() then describes the failure.
+ */
+#ifndef DECAES_AMD_H
+#define DECAES_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dmi_ctx dmi_ctx;
+
+enum {
+    DMI_OK = 0,
+    DMI_EINVAL = -1,       /* an option fails the reference's @assert (message says which) */
+    DMI_EUNSUPPORTED = -2, /* legacy=true, nT2 > 64 or nTE > 124: not handled by this backend */
+    DMI_ENOMEM = -3,
+    DMI_EHIP = -4,         /* HIP runtime error */
+    DMI_ENODEV = -5,       /* no usable gfx950 device */
+    DMI_ECAPACITY = -6     /* a per-voxel device buffer overflowed (reported, never silently truncated) */
+};
+
+/* T2mapOptions.Reg (src/types.jl:66-68) */
+enum { DMI_REG_NONE = 0, DMI_REG_LCURVE = 1, DMI_REG_GCV = 2, DMI_REG_CHI2 = 3, DMI_REG_MDP = 4 };
+
+/* Arithmetic of the voxel path: FP64 everywhere, or FP32 EPG bases with FP64 NNLS/regularization. */
+enum { DMI_PREC_FP64 = 0, DMI_PREC_MIXED = 1 };
+
+/* T2mapOptions (src/types.jl:19-100). Optional reals use NaN for `nothing`. */
+typedef struct {
+    int64_t MatrixSize[3];
+    int32_t nTE;
+    int32_t nT2;
+    double TE;
+    double T2Range[2];
+    double T1;
+    double Threshold; /* -Inf processes every voxel */
+    double MinRefAngle;
+    int32_t nRefAngles;
+    int32_t nRefAnglesMin;
+    int32_t Reg;
+    int32_t legacy;
+    double Chi2Factor;
+    double NoiseLevel;
+    double RefConAngle;
+    double SetFlipAngle;
+    int32_t SaveResidualNorm;
+    int32_t SaveDecayCurve;
+    int32_t SaveRegParam;
+    int32_t SaveNNLSBasis;
+    int32_t precision; /* DMI_PREC_* */
+    int32_t _reserved;
+} dmi_t2map_opts;
+
+/* T2Maps + T2Distributions fields (src/T2mapSEcorr.jl:2-19,64-72). Caller-allocated, column-major, and
+ * pre-filled with NaN: voxels at or below Threshold are left untouched. NULL disables an optional map.
+ * decaybasis is (MatrixSize..., nTE, nT2); with SetFlipAngle it is the single nTE x nT2 basis. */
+typedef struct {
+    double *dist;                     /* (X,Y,Z,nT2) */
+    double *gdn, *ggm, *gva, *fnr, *snr, *alpha;
+    double *resnorm, *decaycurve, *mu, *chi2factor, *decaybasis;
+} dmi_t2map_out;
+
+/* T2partOptions (src/types.jl:139-172). Sigmoid = NaN for `nothing`. */
+typedef struct {
+    int64_t MatrixSize[3];
+    int32_t nT2;
+    int32_t legacy;
+    double T2Range[2];
+    double SPWin[2];
+    double MPWin[2];
+    double Sigmoid;
+} dmi_t2part_opts;
+
+/* Work counters of the last dmi_t2map_* call on this context (device-side tallies). */
+typedef struct {
+    int64_t voxels;        /* voxels above Threshold that were processed */
+    int64_t nnls_solves;   /* Lawson-Hanson solves (flip search + regularization) */
+    int64_t fp64_flops;    /* algorithmic FP64 flops (formula table in DESIGN.md) */
+    int64_t fp32_flops;    /* algorithmic FP32 flops (mixed precision EPG) */
+    double kernel_ms;      /* device time of the voxel kernel, HIP events */
+    double total_ms;       /* device time of the whole call */
+} dmi_stats;
+
+/* Context: one HIP stream on `device`, reusable device buffers and cached grid bases.
+ * Replaces the per-task thread_buffer_maker (src/T2mapSEcorr.jl:596-614). flags: reserved, pass 0. */
+int dmi_create(dmi_ctx **out, int device, uint32_t flags);
+int dmi_destroy(dmi_ctx *ctx);
+/* Error string of the last failed call on ctx (or of dmi_create when ctx is NULL). */
+const char *dmi_last_error(const dmi_ctx *ctx);
+
+/* T2mapSEcorr!(maps, dist, image, opts) (src/T2mapSEcorr.jl:151-196) with host arrays.
+ * image: (X,Y,Z,nTE). alpha_in: the B1 map passed through load_B1map! (src/T2mapSEcorr.jl:56-61), or NULL. */
+int dmi_t2map_f64(dmi_ctx *ctx, const double *image, const dmi_t2map_opts *opts, const double *alpha_in,
+                  const dmi_t2map_out *out);
+/* Same with every array already in device memory (hipMalloc'd on ctx's device); runs on ctx's stream and
+ * returns after the stream has finished. */
+int dmi_t2map_f64_device(dmi_ctx *ctx, const double *d_image, const dmi_t2map_opts *opts, const double *d_alpha_in,
+                         const dmi_t2map_out *d_out);
+
+/* maps["decaybasisset"] (src/T2mapSEcorr.jl:31-33): nTE x nT2 x nRefAngles, or nTE x nT2 with SetFlipAngle. */
+int dmi_epg_basis_set(dmi_ctx *ctx, const dmi_t2map_opts *opts, double *out);
+
+/* T2partSEcorr(dist, opts) (src/T2partSEcorr.jl:40-71): dist (X,Y,Z,nT2) -> sfr, sgm, mfr, mgm (X,Y,Z),
+ * caller-allocated and NaN-filled. Host and device-pointer forms. */
+int dmi_t2part_f64(dmi_ctx *ctx, const double *dist, const dmi_t2part_opts *opts, double *sfr, double *sgm,
+                   double *mfr, double *mgm);
+int dmi_t2part_f64_device(dmi_ctx *ctx, const double *d_dist, const dmi_t2part_opts *opts, double *d_sfr,
+                          double *d_sgm, double *d_mfr, double *d_mgm);
+
+int dmi_last_stats(const dmi_ctx *ctx, dmi_stats *out);
+
+/* Seeded synthetic multi-echo spin-echo volume (host memory, no device work): per voxel a sum of
+ * constant-flip EPG compartments plus Rician noise, zero outside an ellipsoid mask. Counter-based RNG keyed
+ * by (seed, voxel), so any thread count gives the same bits. ncomp = 3 or 4 (the fourth is T2 ~ U(35,50) ms).
+ * Replaces nothing in the reference's hot path; it is the input generator modelled on mock_image
+ * (src/utils.jl:623-658). Returns the number of voxels inside the mask. */
+int64_t dmi_synth_mse(double *image, int64_t nx, int64_t ny, int64_t nz, int32_t nTE, double TE, double T1,
+                      int32_t ncomp, double mask_fraction, uint64_t seed, int32_t nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1760 @@
+/*
+ * decaes_oracle.c -- FP64 CPU restatement of DECAES.jl v0.6.1's T2mapSEcorr / T2partSEcorr path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker and the timed CPU baseline. Nothing in the product library
+ * (decaes.jl_amd/) links, loads or calls this file. It is pinned by tests/test_oracle_*.py against closed
+ * forms, brute force, scipy and the reference's own property tests.
+ *
+ * Conventions: column-major storage as in Julia; indices inside restated loops are kept 1-based through
+ * the IX/AX macros so each loop reads like the cited reference loop. Compiled with -ffp-contract=off;
+ * fma() appears exactly where the reference writes muladd or sits under @muladd (src/NNLS.jl:47).
+ * The reference's @simd reductions may be reassociated by Julia; here every reduction is sequential,
+ * so agreement with Julia is at the rounding level, not bitwise.
+ */
+#define _GNU_SOURCE
+#include "decaes_oracle.h"
+
+#include <complex.h>
+#undef I
+#define IU _Complex_I
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+/* The reference's @simd loops allow Julia to reassociate reductions; the same loops here are OpenMP SIMD
+ * reductions, so the CPU baseline vectorizes where the reference does. */
+#define DO_PRAGMA(x) _Pragma(#x)
+#define SIMD_RED(v) DO_PRAGMA(omp simd reduction(+ : v))
+
+#define EPS64 2.220446049250313e-16
+#define SQRT_EPS64 1.4901161193847656e-8
+#define GOLDEN 1.618033988749895 /* Float64(Base.MathConstants.φ) */
+#define DEG2RAD_C (3.141592653589793 / 180.0)
+
+/* ------------------------------------------------------------------------------------------------
+ * Julia scalar semantics
+ * ------------------------------------------------------------------------------------------------ */
+static inline double jl_min(double a, double b) { return (isnan(a) || isnan(b)) ? NAN : (a < b ? a : b); }
+static inline double jl_max(double a, double b) { return (isnan(a) || isnan(b)) ? NAN : (a > b ? a : b); }
+static inline double jl_sign(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : x); }
+/* isapprox(x, y) with default rtol = sqrt(eps), atol = 0 */
+static inline int jl_isapprox(double x, double y) {
+    if (x == y) return 1;
+    if (!isfinite(x) || !isfinite(y)) return 0;
+    return fabs(x - y) <= SQRT_EPS64 * fmax(fabs(x), fabs(y));
+}
+/* isless ordering used by findmax: NaN is larger than everything */
+static inline int jl_isless(double a, double b) {
+    if (isnan(a)) return 0;
+    if (isnan(b)) return 1;
+    return a < b;
+}
+static inline double norm2(double a, double b) {
+    if (isnan(a) || isnan(b)) return NAN;
+    if (isinf(a) || isinf(b)) return INFINITY;
+    return sqrt(a * a + b * b);
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Grids: logrange (src/utils.jl:7), range(MinRefAngle, 180, length) (src/types.jl:109).
+ * Julia evaluates ranges in TwicePrecision; long double interpolation rounds to the same doubles.
+ * ------------------------------------------------------------------------------------------------ */
+void orc_linrange(double a, double b, int n, double *out) {
+    if (n == 1) { out[0] = a; return; }
+    for (int j = 0; j < n; ++j) {
+        long double t = ((long double)(n - 1 - j) * a + (long double)j * b) / (long double)(n - 1);
+        out[j] = (double)t;
+    }
+    out[0] = a;
+    out[n - 1] = b;
+}
+
+void orc_logrange(double a, double b, int n, double *out) {
+    double la = log(a), lb = log(b);
+    orc_linrange(la, lb, n, out);
+    for (int j = 0; j < n; ++j) out[j] = exp(out[j]);
+    out[0] = a;
+    out[n - 1] = b;
+}
+
+/* sind/cosd with exact reduction in degrees (Julia Base.Math.sind/cosd). */
+static long double deg2rad_extl(double x) { return (long double)x * (3.14159265358979323846264338327950288L / 180.0L); }
+double orc_sind(double x) {
+    if (isinf(x) || isnan(x)) return NAN;
+    double rx = copysign(fmod(x, 360.0), x), arx = fabs(rx);
+    if (rx == 0) return rx;
+    if (arx < 45.0) return (double)sinl(deg2rad_extl(rx));
+    if (arx <= 135.0) return copysign((double)cosl(deg2rad_extl(90.0 - arx)), rx);
+    if (arx == 180.0) return copysign(0.0, rx);
+    if (arx < 225.0) return (double)sinl(deg2rad_extl((180.0 - arx) * jl_sign(rx)));
+    if (arx <= 315.0) return -copysign((double)cosl(deg2rad_extl(270.0 - arx)), rx);
+    return (double)sinl(deg2rad_extl(rx - copysign(360.0, rx)));
+}
+double orc_cosd(double x) {
+    if (isinf(x) || isnan(x)) return NAN;
+    double rx = fabs(fmod(x, 360.0));
+    if (rx <= 45.0) return (double)cosl(deg2rad_extl(rx));
+    if (rx < 135.0) return (double)sinl(deg2rad_extl(90.0 - rx));
+    if (rx <= 225.0) return -(double)cosl(deg2rad_extl(180.0 - rx));
+    if (rx < 315.0) return (double)sinl(deg2rad_extl(rx - 270.0));
+    return (double)cosl(deg2rad_extl(360.0 - rx));
+}
+
+/* erfinv: Winitzki's closed-form start refined by Halley steps on libm erf (own method, not a copy). */
+double orc_erfinv(double y) {
+    if (y <= -1.0) return y == -1.0 ? -INFINITY : NAN;
+    if (y >= 1.0) return y == 1.0 ? INFINITY : NAN;
+    if (y == 0.0) return y;
+    const double a = 0.147, pi = 3.141592653589793;
+    double l = log1p(-y * y), t = 2.0 / (pi * a) + 0.5 * l;
+    double x = copysign(sqrt(sqrt(t * t - l / a) - t), y);
+    for (int it = 0; it < 4; ++it) {
+        double err = erf(x) - y;
+        double d = 2.0 / sqrt(pi) * exp(-x * x);
+        double step = err / d;
+        x -= step / (1.0 + x * step); /* Halley: f''/f' = -2x */
+    }
+    return x;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * EPG decay curves (src/EPGdecaycurve.jl)
+ * ------------------------------------------------------------------------------------------------ */
+
+/* epg_impulse_response! for EPGConstantFlipAngleOptions, DualFlat workspace (:948-1028), then the
+ * |sind(alpha/2) * h| scaling of epg_decay_curve! (:936-946). */
+void orc_epg_const(double *dc, int ETL, double alpha_deg, double TE, double T2, double T1) {
+    double *W1 = calloc((size_t)3 * ETL + 3, sizeof(double)), *W2 = calloc((size_t)3 * ETL + 3, sizeof(double));
+    double *M1 = W1 - 1, *M2 = W2 - 1; /* 1-based views */
+    double *D = dc - 1;
+    const int Dy = ETL, Dz = 2 * ETL;
+    double al = alpha_deg * DEG2RAD_C;
+    double E1 = exp(-(TE / 2) / T1), E2 = exp(-(TE / 2) / T2);
+    double sn = sin(al), cs = cos(al);
+    double E2h = (E2 * E2) / 2, E1E2 = E1 * E2, E1sq = E1 * E1;
+    double a = E2h, b = E2h * cs, c = E1E2 * sn, d = E1sq * cs, cp = -c / 2;
+    double *t;
+
+    D[1] = a - b;
+    M1[1] = a - b; M1[1 + Dy] = 0; M1[1 + Dz] = cp;
+    M1[2] = a + b; M1[2 + Dy] = 0; M1[2 + Dz] = 0;
+    t = M1; M1 = M2; M2 = t;
+
+    for (int i = 2; i <= ETL / 2; ++i) {
+        double F = M2[1], Fb = M2[1 + Dy], Z = M2[1 + Dz];
+        double C = F + Fb, S = F - Fb, Cp = a * C, Sp = b * S;
+        D[i] = M1[1] = fma(-c, Z, Cp - Sp);
+        M1[2] = fma(c, Z, Cp + Sp);
+        M1[1 + Dz] = fma(cp, S, d * Z);
+        for (int k = 2; k <= i - 1; ++k) {
+            F = M2[k]; Fb = M2[k + Dy]; Z = M2[k + Dz];
+            C = F + Fb; S = F - Fb; Cp = a * C; Sp = b * S;
+            M1[k + 1] = fma(c, Z, Cp + Sp);
+            M1[k - 1 + Dy] = fma(-c, Z, Cp - Sp);
+            M1[k + Dz] = fma(cp, S, d * Z);
+        }
+        F = M2[i]; Fb = M2[i + Dy]; Z = M2[i + Dz];
+        C = F + Fb; S = F - Fb; Cp = a * C; Sp = b * S;
+        M1[i + 1] = fma(c, Z, Cp + Sp);
+        M1[i - 1 + Dy] = fma(-c, Z, Cp - Sp);
+        M1[i + Dz] = fma(cp, S, d * Z);
+        M1[i + Dy] = 0; M1[i + 1 + Dy] = 0; M1[i + 1 + Dz] = 0;
+        t = M1; M1 = M2; M2 = t;
+    }
+    for (int i = ETL / 2 + 1; i <= ETL - 1; ++i) {
+        double F = M2[1], Fb = M2[1 + Dy], Z = M2[1 + Dz];
+        double C = F + Fb, S = F - Fb, Cp = a * C, Sp = b * S;
+        D[i] = M1[1] = fma(-c, Z, Cp - Sp);
+        M1[2] = fma(c, Z, Cp + Sp);
+        M1[1 + Dz] = fma(cp, S, d * Z);
+        for (int k = 2; k <= ETL - i + 1; ++k) {
+            F = M2[k]; Fb = M2[k + Dy]; Z = M2[k + Dz];
+            C = F + Fb; S = F - Fb; Cp = a * C; Sp = b * S;
+            M1[k + 1] = fma(c, Z, Cp + Sp);
+            M1[k - 1 + Dy] = fma(-c, Z, Cp - Sp);
+            M1[k + Dz] = fma(cp, S, d * Z);
+        }
+        t = M1; M1 = M2; M2 = t;
+    }
+    {
+        double F = M2[1], Fb = M2[1 + Dy], Z = M2[1 + Dz];
+        double C = F + Fb, S = F - Fb;
+        D[ETL] = fma(-c, Z, fma(a, C, -b * S));
+    }
+    double m0 = orc_sind(alpha_deg / 2);
+    for (int i = 1; i <= ETL; ++i) D[i] = fabs(m0 * D[i]);
+    free(W1);
+    free(W2);
+}
+
+/* Forward-mode dual numbers: what ForwardDiff.Dual{1} computes (value, d/dalpha). */
+typedef struct { double v, d; } dl;
+static inline dl dl_c(double v) { dl r = {v, 0}; return r; }
+static inline dl dl_add(dl a, dl b) { dl r = {a.v + b.v, a.d + b.d}; return r; }
+static inline dl dl_sub(dl a, dl b) { dl r = {a.v - b.v, a.d - b.d}; return r; }
+static inline dl dl_neg(dl a) { dl r = {-a.v, -a.d}; return r; }
+static inline dl dl_mul(dl a, dl b) { dl r = {a.v * b.v, a.d * b.v + a.v * b.d}; return r; }
+static inline dl dl_scale(dl a, double s) { dl r = {a.v * s, a.d * s}; return r; }
+static inline dl dl_div(dl a, double s) { dl r = {a.v / s, a.d / s}; return r; }
+static inline dl dl_fma(dl a, dl b, dl c) { dl r = {fma(a.v, b.v, c.v), a.d * b.v + a.v * b.d + c.d}; return r; }
+static inline dl dl_abs(dl a) { return signbit(a.v) ? dl_neg(a) : a; }
+static inline dl dl_dot3(const dl *x, const dl *y) { return dl_add(dl_add(dl_mul(x[0], y[0]), dl_mul(x[1], y[1])), dl_mul(x[2], y[2])); }
+
+/* Constant-flip recursion (:948-1028) carried out on duals seeded with d(alpha)=1. */
+static void epg_const_dual(double *dc, double *ddc, int ETL, double alpha_deg, double TE, double T2, double T1) {
+    dl *W1 = calloc((size_t)3 * ETL + 3, sizeof(dl)), *W2 = calloc((size_t)3 * ETL + 3, sizeof(dl));
+    dl *M1 = W1 - 1, *M2 = W2 - 1, *t;
+    dl *D = calloc((size_t)ETL + 1, sizeof(dl));
+    const int Dy = ETL, Dz = 2 * ETL;
+    dl al = {alpha_deg * DEG2RAD_C, DEG2RAD_C};
+    double E1 = exp(-(TE / 2) / T1), E2 = exp(-(TE / 2) / T2);
+    dl sn = {sin(al.v), cos(al.v) * al.d}, cs = {cos(al.v), -sin(al.v) * al.d};
+    double E2h = (E2 * E2) / 2, E1E2 = E1 * E2, E1sq = E1 * E1;
+    dl a = dl_c(E2h), b = dl_scale(cs, E2h), c = dl_scale(sn, E1E2), d = dl_scale(cs, E1sq), cp = dl_div(dl_neg(c), 2);
+    dl mc = dl_neg(c), zero = dl_c(0);
+
+    D[1] = dl_sub(a, b);
+    M1[1] = dl_sub(a, b); M1[1 + Dy] = zero; M1[1 + Dz] = cp;
+    M1[2] = dl_add(a, b); M1[2 + Dy] = zero; M1[2 + Dz] = zero;
+    t = M1; M1 = M2; M2 = t;
+#define EPG_CSTEP(k)                                                                                              \
+    F = M2[k]; Fb = M2[(k) + Dy]; Z = M2[(k) + Dz];                                                               \
+    C = dl_add(F, Fb); S = dl_sub(F, Fb); Cp = dl_mul(a, C); Sp = dl_mul(b, S);
+    for (int i = 2; i <= ETL / 2; ++i) {
+        dl F, Fb, Z, C, S, Cp, Sp;
+        EPG_CSTEP(1);
+        D[i] = M1[1] = dl_fma(mc, Z, dl_sub(Cp, Sp));
+        M1[2] = dl_fma(c, Z, dl_add(Cp, Sp));
+        M1[1 + Dz] = dl_fma(cp, S, dl_mul(d, Z));
+        for (int k = 2; k <= i; ++k) {
+            EPG_CSTEP(k);
+            M1[k + 1] = dl_fma(c, Z, dl_add(Cp, Sp));
+            M1[k - 1 + Dy] = dl_fma(mc, Z, dl_sub(Cp, Sp));
+            M1[k + Dz] = dl_fma(cp, S, dl_mul(d, Z));
+        }
+        M1[i + Dy] = zero; M1[i + 1 + Dy] = zero; M1[i + 1 + Dz] = zero;
+        t = M1; M1 = M2; M2 = t;
+    }
+    for (int i = ETL / 2 + 1; i <= ETL - 1; ++i) {
+        dl F, Fb, Z, C, S, Cp, Sp;
+        EPG_CSTEP(1);
+        D[i] = M1[1] = dl_fma(mc, Z, dl_sub(Cp, Sp));
+        M1[2] = dl_fma(c, Z, dl_add(Cp, Sp));
+        M1[1 + Dz] = dl_fma(cp, S, dl_mul(d, Z));
+        for (int k = 2; k <= ETL - i + 1; ++k) {
+            EPG_CSTEP(k);
+            M1[k + 1] = dl_fma(c, Z, dl_add(Cp, Sp));
+            M1[k - 1 + Dy] = dl_fma(mc, Z, dl_sub(Cp, Sp));
+            M1[k + Dz] = dl_fma(cp, S, dl_mul(d, Z));
+        }
+        t = M1; M1 = M2; M2 = t;
+    }
+    {
+        dl F, Fb, Z, C, S, Cp, Sp;
+        EPG_CSTEP(1);
+        (void)Cp; (void)Sp;
+        D[ETL] = dl_fma(mc, Z, dl_fma(a, C, dl_neg(dl_mul(b, S))));
+    }
+#undef EPG_CSTEP
+    /* m0 = sind(alpha/2); ForwardDiff differentiates sind as deg2rad(cosd(x)) */
+    dl m0 = {orc_sind(alpha_deg / 2), orc_cosd(alpha_deg / 2) * DEG2RAD_C * 0.5};
+    for (int i = 1; i <= ETL; ++i) {
+        dl r = dl_abs(dl_mul(m0, D[i]));
+        dc[i - 1] = r.v;
+        if (ddc) ddc[i - 1] = r.d;
+    }
+    free(W1); free(W2); free(D);
+}
+
+/* epg_decay_curve! for EPGOptions (general beta), DualVector_Split_Dynamic (:722-818), on duals. */
+static void epg_general_dual(double *dc, double *ddc, int ETL, double alpha_deg, double TE, double T2, double T1,
+                             double beta, double seed) {
+    dl *W1 = calloc((size_t)3 * (ETL + 2), sizeof(dl)), *W2 = calloc((size_t)3 * (ETL + 2), sizeof(dl));
+    dl *P1 = W1, *P2 = W2, *t; /* MPSV[j] at P[3*(j-1) .. 3*(j-1)+2] */
+#define MP(P, j) ((P) + 3 * ((j)-1))
+    dl alpha = {alpha_deg, seed};
+    dl A = dl_div(alpha, 180.0);
+    dl a1r = dl_scale(dl_scale(A, 180.0), DEG2RAD_C), air = dl_scale(dl_scale(A, beta), DEG2RAD_C);
+    double E1 = exp(-(TE / 2) / T1), E2 = exp(-(TE / 2) / T2);
+    dl h = dl_div(a1r, 2.0);
+    dl s_h = {sin(h.v), cos(h.v) * h.d}, c_h = {cos(h.v), -sin(h.v) * h.d};
+    dl s2_h = dl_mul(s_h, s_h), c2_h = dl_mul(c_h, c_h);
+    dl sina1 = dl_mul(dl_scale(s_h, 2.0), c_h);
+    dl sinai = {sin(air.v), cos(air.v) * air.d}, cosai = {cos(air.v), -sin(air.v) * air.d};
+    dl c2_hi = dl_div(dl_add(dl_c(1.0), cosai), 2.0);
+    dl s2_hi = dl_sub(dl_c(1.0), c2_hi);
+    double E2sq = E2 * E2, E1E2 = E1 * E2, E1sq = E1 * E1;
+    dl a1 = dl_scale(c2_h, E2sq), b1 = dl_scale(s2_h, E2sq), c1 = dl_scale(sina1, E1E2);
+    dl ai = dl_scale(c2_hi, E2sq), bi = dl_scale(s2_hi, E2sq), ci = dl_scale(sinai, E1E2), di = dl_scale(cosai, E1sq);
+    dl Fv[3] = {ai, bi, ci}, Fbv[3] = {bi, ai, dl_neg(ci)}, Zv[3] = {dl_div(dl_neg(ci), 2.0), dl_div(ci, 2.0), di};
+    dl zero = dl_c(0);
+    dl *D = calloc((size_t)ETL + 1, sizeof(dl));
+
+    /* i = 1 */
+    dl m0 = s_h;
+    MP(P1, 1)[0] = dl_mul(b1, m0); MP(P1, 1)[1] = zero; MP(P1, 1)[2] = dl_div(dl_mul(dl_neg(c1), m0), 2.0);
+    MP(P1, 2)[0] = dl_mul(a1, m0); MP(P1, 2)[1] = zero; MP(P1, 2)[2] = zero;
+    D[1] = dl_abs(MP(P1, 1)[0]);
+    t = P1; P1 = P2; P2 = t;
+    /* i = 2 */
+    {
+        dl *M0 = MP(P2, 1), *M1 = MP(P2, 2);
+        dl FM0 = dl_dot3(Fv, M0), FbM0 = dl_dot3(Fbv, M0), ZM0 = dl_dot3(Zv, M0);
+        dl FM1 = dl_dot3(Fv, M1), FbM1 = dl_dot3(Fbv, M1), ZM1 = dl_dot3(Zv, M1);
+        dl r1[3] = {FbM0, FbM1, ZM0}, r2[3] = {FM0, zero, ZM1}, r3[3] = {FM1, zero, zero};
+        memcpy(MP(P1, 1), r1, sizeof r1); memcpy(MP(P1, 2), r2, sizeof r2); memcpy(MP(P1, 3), r3, sizeof r3);
+        D[2] = dl_abs(FbM0);
+        t = P1; P1 = P2; P2 = t;
+    }
+    for (int i = 3; i <= ETL - 1; ++i) {
+        int firsthalf = i <= ETL / 2;
+        dl *M0 = MP(P2, 1), *M1 = MP(P2, 2), *M2 = MP(P2, 3);
+        dl FM0 = dl_dot3(Fv, M0), FbM0 = dl_dot3(Fbv, M0), ZM0 = dl_dot3(Zv, M0);
+        dl FM1 = dl_dot3(Fv, M1), FbM1 = dl_dot3(Fbv, M1), ZM1 = dl_dot3(Zv, M1);
+        dl FM2 = dl_dot3(Fv, M2), FbM2 = dl_dot3(Fbv, M2), ZM2 = dl_dot3(Zv, M2);
+        dl r1[3] = {FbM0, FbM1, ZM0}, r2[3] = {FM0, FbM2, ZM1};
+        memcpy(MP(P1, 1), r1, sizeof r1); memcpy(MP(P1, 2), r2, sizeof r2);
+        int jend = firsthalf ? i - 1 : ETL - i;
+        for (int j = 3; j <= jend; ++j) {
+            FM0 = FM1; FM1 = FM2; ZM1 = ZM2;
+            dl *Mj = MP(P2, j + 1);
+            FM2 = dl_dot3(Fv, Mj); FbM2 = dl_dot3(Fbv, Mj); ZM2 = dl_dot3(Zv, Mj);
+            dl r[3] = {FM0, FbM2, ZM1};
+            memcpy(MP(P1, j), r, sizeof r);
+        }
+        if (firsthalf) {
+            dl ri[3] = {FM1, zero, ZM2}, ri1[3] = {FM2, zero, zero};
+            memcpy(MP(P1, i), ri, sizeof ri); memcpy(MP(P1, i + 1), ri1, sizeof ri1);
+        }
+        D[i] = dl_abs(FbM0);
+        t = P1; P1 = P2; P2 = t;
+    }
+    D[ETL] = dl_abs(dl_dot3(Fbv, MP(P2, 1)));
+    for (int i = 1; i <= ETL; ++i) {
+        dc[i - 1] = D[i].v;
+        if (ddc) ddc[i - 1] = D[i].d;
+    }
+#undef MP
+    free(W1); free(W2); free(D);
+}
+
+void orc_epg_general(double *dc, int ETL, double alpha, double TE, double T2, double T1, double beta) {
+    epg_general_dual(dc, NULL, ETL, alpha, TE, T2, T1, beta, 0.0);
+}
+
+void orc_epg_dalpha(double *dc, double *ddc, int ETL, double alpha, double TE, double T2, double T1, double beta) {
+    if (beta == 180.0)
+        epg_const_dual(dc, ddc, ETL, alpha, TE, T2, T1);
+    else
+        epg_general_dual(dc, ddc, ETL, alpha, TE, T2, T1, beta, 1.0);
+}
+
+/* EPGWork_Basic_Cplx (:271-322) with element_flipmat (:6-9): the reference's readable test reference. */
+void orc_epg_basic(double *dc, int ETL, double alpha, double TE, double T2, double T1, double beta) {
+    double A = alpha / 180, aex = A * 90, a1 = A * 180, ai = A * beta;
+    double E1 = exp(-(TE / 2) / T1), E2 = exp(-(TE / 2) / T2);
+    double Ev[3] = {E2, E2, E1};
+    double complex R1[3][3], Ri[3][3];
+    double complex(*Rs[2])[3] = {R1, Ri};
+    double angs[2] = {a1, ai};
+    for (int r = 0; r < 2; ++r) {
+        double x = angs[r], c2 = orc_cosd(x / 2) * orc_cosd(x / 2), s2 = orc_sind(x / 2) * orc_sind(x / 2), s = orc_sind(x);
+        Rs[r][0][0] = c2; Rs[r][0][1] = s2; Rs[r][0][2] = -IU * s;
+        Rs[r][1][0] = s2; Rs[r][1][1] = c2; Rs[r][1][2] = IU * s;
+        Rs[r][2][0] = -IU * s / 2; Rs[r][2][1] = IU * s / 2; Rs[r][2][2] = orc_cosd(x);
+    }
+    double complex(*M)[3] = calloc((size_t)ETL, sizeof(*M));
+    M[0][0] = orc_sind(aex);
+    for (int i = 0; i < ETL; ++i) {
+        double complex(*R)[3] = (i == 0) ? R1 : Ri;
+        for (int j = 0; j < ETL; ++j) {
+            double complex v[3], o[3];
+            for (int r = 0; r < 3; ++r) v[r] = Ev[r] * M[j][r];
+            for (int r = 0; r < 3; ++r) o[r] = R[r][0] * v[0] + R[r][1] * v[1] + R[r][2] * v[2];
+            for (int r = 0; r < 3; ++r) M[j][r] = o[r];
+        }
+        if (ETL >= 2) {
+            double complex Mi[3], Mi1[3], Mim1[3];
+            memcpy(Mi, M[0], sizeof Mi);
+            memcpy(Mi1, M[1], sizeof Mi1);
+            M[0][0] = Mi[1]; M[0][1] = Mi1[1]; M[0][2] = Mi[2];
+            for (int j = 1; j < ETL - 1; ++j) {
+                memcpy(Mim1, Mi, sizeof Mi);
+                memcpy(Mi, Mi1, sizeof Mi);
+                memcpy(Mi1, M[j + 1], sizeof Mi1);
+                M[j][0] = Mim1[0]; M[j][1] = Mi1[1]; M[j][2] = Mi[2];
+            }
+            memcpy(Mim1, Mi, sizeof Mi);
+            memcpy(Mi, Mi1, sizeof Mi);
+            M[ETL - 1][0] = Mim1[0]; M[ETL - 1][1] = 0; M[ETL - 1][2] = Mi[2];
+        }
+        for (int j = 0; j < ETL; ++j)
+            for (int r = 0; r < 3; ++r) M[j][r] = Ev[r] * M[j][r];
+        dc[i] = cabs(M[0][0]);
+    }
+    free(M);
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Lawson-Hanson NNLS (src/NNLS.jl). Workspace A is M x N column-major with leading dimension M.
+ * ------------------------------------------------------------------------------------------------ */
+typedef struct {
+    int M, N;
+    double *A, *b, *x, *w, *zz;
+    int *idx, *invidx;
+    unsigned char *diag;
+    double rnorm;
+    int mode, nsetp;
+} nnls_ws;
+
+#define AX(i, j) A[((size_t)(i)-1) + ((size_t)(j)-1) * (size_t)lda]
+
+static void nnls_ws_init(nnls_ws *ws, int M, int N) {
+    ws->M = M; ws->N = N;
+    ws->A = calloc((size_t)M * N, sizeof(double));
+    ws->b = calloc((size_t)M, sizeof(double));
+    ws->x = calloc((size_t)N, sizeof(double));
+    ws->w = calloc((size_t)N, sizeof(double));
+    ws->zz = calloc((size_t)M, sizeof(double));
+    ws->idx = calloc((size_t)N, sizeof(int));
+    ws->invidx = calloc((size_t)N, sizeof(int));
+    ws->diag = calloc((size_t)N, 1);
+    ws->rnorm = 0; ws->mode = 0; ws->nsetp = 0;
+}
+static void nnls_ws_free(nnls_ws *ws) {
+    free(ws->A); free(ws->b); free(ws->x); free(ws->w); free(ws->zz);
+    free(ws->idx); free(ws->invidx); free(ws->diag);
+}
+
+/* construct_apply_householder! (:259-336). Returns tau (< 0 rejects the column). */
+static double construct_apply_householder(double *A, int lda, double *b1b, int ip, int jp, int m) {
+    double *b = b1b - 1;
+    if (ip > m) return 0.0;
+    double alpha = AX(ip, jp);
+    double xnorm = 0.0;
+    SIMD_RED(xnorm)
+    for (int i = ip; i <= m; ++i) xnorm = fma(AX(i, jp), AX(i, jp), xnorm);
+    xnorm = sqrt(xnorm);
+    if (xnorm == 0) return -1.0;
+    double beta = copysign(xnorm, alpha);
+    alpha = alpha + beta;
+    double tau = alpha / beta;
+    double sm = b[ip];
+    SIMD_RED(sm)
+    for (int i = ip + 1; i <= m; ++i) sm = fma(b[i], AX(i, jp) / alpha, sm);
+    sm *= -tau;
+    double A1 = -beta;
+    double b1 = b[ip] + sm;
+    if (b1 / A1 > 0) {
+        if (ip < m) {
+            if (ip != jp) {
+                for (int i = 1; i <= ip - 1; ++i) { double tmp = AX(i, ip); AX(i, ip) = AX(i, jp); AX(i, jp) = tmp; }
+                b[ip] = b1;
+                { double tmp = AX(ip, ip); AX(ip, ip) = A1; AX(ip, jp) = tmp; }
+                for (int i = ip + 1; i <= m; ++i) {
+                    double Aij = AX(i, jp) / alpha;
+                    double tmp = AX(i, ip);
+                    AX(i, ip) = Aij; AX(i, jp) = tmp;
+                    b[i] = fma(sm, Aij, b[i]);
+                }
+            } else {
+                b[ip] = b1;
+                AX(ip, ip) = A1;
+                for (int i = ip + 1; i <= m; ++i) {
+                    double Aii = AX(i, ip) / alpha;
+                    AX(i, ip) = Aii;
+                    b[i] = fma(sm, Aii, b[i]);
+                }
+            }
+        } else {
+            tau = 0.0;
+            if (ip != jp)
+                for (int i = 1; i <= m; ++i) { double tmp = AX(i, ip); AX(i, ip) = AX(i, jp); AX(i, jp) = tmp; }
+        }
+        return tau;
+    }
+    return -1.0;
+}
+
+/* apply_householder_dual! (:379-436): apply the reflector in column j1 to columns j1+1..n and
+ * recompute the dual w_j = sum_{i>j1} A_ij b_i on the transformed rows. */
+static void apply_householder_dual(double *A, int lda, double *w1, const double *b1b, double tau, int j1, int m1,
+                                   int n) {
+    double *w = w1 - 1;
+    const double *b = b1b - 1;
+    if (j1 >= m1) return;
+    double aii = AX(j1, j1);
+    AX(j1, j1) = 1;
+    for (int j = j1 + 1; j <= n; ++j) {
+        double sm = 0.0;
+        SIMD_RED(sm)
+        for (int i = j1; i <= m1; ++i) sm = fma(AX(i, j), AX(i, j1), sm);
+        sm *= -tau;
+        double wj = 0.0;
+        AX(j1, j) = AX(j1, j) + sm;
+        SIMD_RED(wj)
+        for (int i = j1 + 1; i <= m1; ++i) {
+            double Aij = fma(sm, AX(i, j1), AX(i, j));
+            wj = fma(Aij, b[i], wj);
+            AX(i, j) = Aij;
+        }
+        w[j] = wj;
+    }
+    AX(j1, j1) = aii;
+}
+
+/* compute_dual! (:441-470) */
+static void compute_dual(double *w1, const double *A, int lda, const double *b1b, int j1, int m1, int n) {
+    double *w = w1 - 1;
+    const double *b = b1b - 1;
+    for (int j = j1; j <= n; ++j) {
+        double sm = 0.0;
+        SIMD_RED(sm)
+        for (int i = j1; i <= m1; ++i) sm = fma(AX(i, j), b[i], sm);
+        w[j] = sm;
+    }
+}
+
+/* solve_triangular_system! (:506-539) */
+static void solve_triangular(double *z1, const double *A, int lda, int n, int transp) {
+    double *z = z1 - 1;
+    if (!transp) {
+        for (int j = n; j >= 1; --j) {
+            double zi = -z[j] / AX(j, j);
+            for (int i = 1; i <= j - 1; ++i) z[i] = fma(AX(i, j), zi, z[i]);
+            z[j] = -zi;
+        }
+    } else {
+        for (int j = 1; j <= n; ++j) {
+            double z1v = z[j];
+            for (int l = 1; l <= j - 1; ++l) z1v = fma(-AX(l, j), z[l], z1v);
+            z1v /= AX(j, j);
+            z[j] = z1v;
+        }
+    }
+}
+
+/* largest_positive_dual (:541-554) */
+static double largest_positive_dual(const double *w1, int j1, int n, int *jmax) {
+    const double *w = w1 - 1;
+    double wmax = 0.0;
+    *jmax = 0;
+    for (int j = j1; j <= n; ++j)
+        if (w[j] > wmax) { wmax = w[j]; *jmax = j; }
+    return wmax;
+}
+
+/* The deletion step shared by both solvers (:731-779 / :967-1015): remove coefficient imv from set P
+ * by Givens rotations and column shifts, then drop any further nonpositive coefficients. */
+static void nnls_delete(double *A, int lda, double *b1b, double *x1, int *idx1, int *pimv, int *pnsetp, int m, int n) {
+    double *b = b1b - 1, *x = x1 - 1;
+    int *idx = idx1 - 1;
+    int imv = *pimv, nsetp = *pnsetp;
+    for (;;) {
+        x[idx[imv]] = 0.0;
+        if (imv != nsetp) {
+            for (int i = imv + 1; i <= nsetp; ++i) {
+                double aa = AX(i - 1, i), bb = AX(i, i);
+                double sig = hypot(aa, bb), cc = aa / sig, ss = bb / sig;
+                AX(i - 1, i) = sig;
+                AX(i, i) = 0.0;
+                for (int j = 1; j <= i - 1; ++j) {
+                    double u = AX(i - 1, j), v = AX(i, j);
+                    AX(i - 1, j) = fma(cc, u, ss * v);
+                    AX(i, j) = fma(-ss, u, cc * v);
+                }
+                for (int j = i + 1; j <= n; ++j) {
+                    double u = AX(i - 1, j), v = AX(i, j);
+                    AX(i - 1, j) = fma(cc, u, ss * v);
+                    AX(i, j) = fma(-ss, u, cc * v);
+                }
+                double u = b[i - 1], v = b[i];
+                b[i - 1] = fma(cc, u, ss * v);
+                b[i] = fma(-ss, u, cc * v);
+            }
+            for (int j = imv; j <= nsetp - 1; ++j) {
+                for (int i = 1; i <= m; ++i) { double tmp = AX(i, j + 1); AX(i, j + 1) = AX(i, j); AX(i, j) = tmp; }
+                int ti = idx[j]; idx[j] = idx[j + 1]; idx[j + 1] = ti;
+            }
+        }
+        nsetp -= 1;
+        int allfeasible = 1;
+        for (int i = 1; i <= nsetp; ++i)
+            if (x[idx[i]] <= 0) { allfeasible = 0; imv = i; break; }
+        if (allfeasible) break;
+    }
+    *pimv = imv;
+    *pnsetp = nsetp;
+}
+
+/* unsafe_nnls! (:605-825) when lam < 0 (plain), unsafe_nnls!(work, λ) (:827-1061) otherwise. */
+static void unsafe_nnls(nnls_ws *ws, int tikh, double lam, int init_dual, int max_iter) {
+    double *A = ws->A, *bb = ws->b, *xx = ws->x, *ww = ws->w, *zzz = ws->zz;
+    int *idxx = ws->idx, *invidxx = ws->invidx;
+    unsigned char *diagg = ws->diag;
+    double *b = bb - 1, *x = xx - 1, *w = ww - 1, *zz = zzz - 1;
+    int *idx = idxx - 1, *invidx = invidxx - 1;
+    unsigned char *diag = diagg - 1;
+    const int lda = ws->M, M = ws->M, N = ws->N;
+    int m = tikh ? M - N : M;
+    const int n = N;
+    if (max_iter < 0) max_iter = 3 * N;
+
+    if (init_dual) {
+        for (int j = 1; j <= n; ++j) w[j] = 0;
+        compute_dual(ww, A, lda, bb, 1, m, n);
+    }
+    int nsetp = 0, iter = 0, terminated = 0;
+    ws->mode = 0;
+    for (;;) {
+        if (tikh ? (nsetp >= n) : (nsetp >= n || nsetp >= m)) { terminated = 1; break; }
+        int jmax = nsetp;
+        double tau = 0.0;
+        for (;;) {
+            double wmax = largest_positive_dual(ww, nsetp + 1, n, &jmax);
+            if (wmax <= 0) { terminated = 1; break; }
+            if (tikh && !diag[idx[jmax]]) AX(m + 1, jmax) = lam;
+            tau = construct_apply_householder(A, lda, bb, nsetp + 1, jmax, tikh ? (m + 1 < M ? m + 1 : M) : m);
+            if (tau >= 0) break;
+            w[jmax] = 0.0;
+            if (tikh && m < M) AX(m + 1, jmax) = 0.0;
+        }
+        if (terminated) break;
+        if (tikh && !diag[idx[jmax]]) {
+            m = (m + 1 < M) ? m + 1 : M;
+            diag[idx[jmax]] = 1;
+        }
+        nsetp += 1;
+        { int t = idx[nsetp]; idx[nsetp] = idx[jmax]; idx[jmax] = t; }
+        if (nsetp < n) apply_householder_dual(A, lda, ww, bb, tau, nsetp, m, n);
+        for (int i = nsetp + 1; i <= m; ++i) AX(i, nsetp) = 0.0;
+        w[nsetp] = 0.0;
+        for (int i = 1; i <= nsetp; ++i) zz[i] = b[i];
+        solve_triangular(zzz, A, lda, nsetp, 0);
+
+        int dual_flag = 0;
+        for (;;) {
+            iter += 1;
+            if (iter > max_iter) { ws->mode = 1; terminated = 1; break; }
+            int imv = nsetp;
+            double alpha = 2.0;
+            for (int i = 1; i <= nsetp; ++i) {
+                if (zz[i] <= 0) {
+                    double xi = x[idx[i]];
+                    double t = -xi / (zz[i] - xi);
+                    if (alpha > t) { imv = i; alpha = t; }
+                }
+            }
+            if (alpha == 2) break;
+            dual_flag = 1;
+            for (int i = 1; i <= nsetp; ++i) {
+                int ix = idx[i];
+                x[ix] = fma(alpha, zz[i] - x[ix], x[ix]);
+            }
+            nnls_delete(A, lda, bb, xx, idxx, &imv, &nsetp, m, n);
+            for (int i = 1; i <= nsetp; ++i) zz[i] = b[i];
+            solve_triangular(zzz, A, lda, nsetp, 0);
+        }
+        if (terminated) break;
+        if (dual_flag) compute_dual(ww, A, lda, bb, nsetp + 1, m, n);
+        for (int i = 1; i <= nsetp; ++i) x[idx[i]] = zz[i];
+    }
+    for (int i = 1; i <= n; ++i) invidx[idx[i]] = i;
+    double sm = 0.0;
+    int Mlim = tikh ? M : m;
+    if (nsetp < Mlim) {
+        SIMD_RED(sm)
+        for (int i = nsetp + 1; i <= Mlim; ++i) { double bi = b[i]; zz[i] = bi; sm = fma(bi, bi, sm); }
+    } else {
+        for (int j = 1; j <= n; ++j) w[j] = 0.0;
+    }
+    ws->rnorm = sqrt(sm);
+    ws->nsetp = nsetp;
+}
+
+/* NNLSProblem solve! with warm start (src/lsqnonneg.jl:30-84), or the Tikhonov form (:86-164) when mu >= 0.
+ * A0 is m x n column-major (lda0 = m). */
+static void nnlsprob_solve(nnls_ws *ws, const double *A0, const double *b0, int m, int n, double mu, int max_iter) {
+    const int tikh = mu >= 0;
+    double *C = ws->A, *f = ws->b, *x = ws->x, *w = ws->w, *z = ws->zz;
+    int *idx = ws->idx;
+    const int ldc = ws->M;
+#define A0X(i, j) A0[((size_t)(i)-1) + ((size_t)(j)-1) * (size_t)m]
+#define CX(i, j) C[((size_t)(i)-1) + ((size_t)(j)-1) * (size_t)ldc]
+    double den = 0.0;
+    SIMD_RED(den)
+    for (int i = 1; i <= m; ++i) den += A0X(i, n) * A0X(i, n);
+    if (tikh) den += mu * mu;
+    double xj = 0.0;
+    SIMD_RED(xj)
+    for (int i = 1; i <= m; ++i) xj += (A0X(i, n) / den) * b0[i - 1];
+    for (int i = 1; i <= m; ++i) z[i - 1] = b0[i - 1] - A0X(i, n) * xj;
+    for (int j = 1; j <= n - 1; ++j) {
+        double wj = 0.0;
+        SIMD_RED(wj)
+        for (int i = 1; i <= m; ++i) {
+            double Aij = A0X(i, j);
+            wj += Aij * z[i - 1];
+            CX(i, j) = Aij;
+        }
+        w[j - 1] = wj;
+    }
+    w[n - 1] = 0;
+    {
+        int allneg = 1;
+        for (int j = 0; j < n; ++j)
+            if (!(w[j] <= 0)) { allneg = 0; break; }
+        w[n - 1] = allneg ? 1.0 : 0.0;
+    }
+    for (int i = 1; i <= m; ++i) { f[i - 1] = b0[i - 1]; CX(i, n) = A0X(i, n); }
+    if (tikh) {
+        for (int j = 1; j <= n; ++j)
+            for (int i = m + 1; i <= ldc; ++i) CX(i, j) = 0;
+    }
+    for (int j = 1; j <= n; ++j) {
+        x[j - 1] = 0;
+        if (tikh) { f[m + j - 1] = 0; ws->diag[j - 1] = 0; }
+        idx[j - 1] = j;
+    }
+#undef A0X
+#undef CX
+    unsafe_nnls(ws, tikh, mu, 0, max_iter);
+}
+
+double orc_nnls(const double *A, const double *b, int m, int n, double *x, int *nsetp, int *mode, int *idx,
+                double *w_out) {
+    nnls_ws ws;
+    nnls_ws_init(&ws, m, n);
+    nnlsprob_solve(&ws, A, b, m, n, -1.0, -1);
+    memcpy(x, ws.x, sizeof(double) * n);
+    if (nsetp) *nsetp = ws.nsetp;
+    if (mode) *mode = ws.mode;
+    if (idx) memcpy(idx, ws.idx, sizeof(int) * n);
+    if (w_out) memcpy(w_out, ws.w, sizeof(double) * n);
+    double r = ws.rnorm;
+    nnls_ws_free(&ws);
+    return r;
+}
+
+double orc_nnls_tikh(const double *A, const double *b, int m, int n, double mu, double *x, int *nsetp, int *mode) {
+    nnls_ws ws;
+    nnls_ws_init(&ws, m + n, n);
+    nnlsprob_solve(&ws, A, b, m, n, mu, -1);
+    memcpy(x, ws.x, sizeof(double) * n);
+    if (nsetp) *nsetp = ws.nsetp;
+    if (mode) *mode = ws.mode;
+    double r = ws.rnorm;
+    nnls_ws_free(&ws);
+    return r;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Per-voxel regularization workspace: the plain NNLSProblem, one Tikhonov workspace and the 8-entry
+ * NNLSTikhonovRegProblemCache (src/lsqnonneg.jl:392-444). A cache entry keeps what later reads of
+ * that entry need (mu, x, rnorm, nsetp, idx); re-solving a mu gives the same bits, so the cache only
+ * removes repeated solves and has no effect on results.
+ * ------------------------------------------------------------------------------------------------ */
+#define TIKH_CACHE_N 8
+typedef struct {
+    int m, n;
+    const double *A, *b;
+    nnls_ws plain, tikh;
+    double cmu[TIKH_CACHE_N], crnorm[TIKH_CACHE_N];
+    int cnsetp[TIKH_CACHE_N];
+    double *cx;  /* TIKH_CACHE_N x n */
+    int *cidx;   /* TIKH_CACHE_N x n */
+    int ccur;    /* 0-based current entry */
+    double *gamma;
+    int ngamma;
+    long nsolves;
+} reg_ws;
+
+static void reg_ws_init(reg_ws *R, int m, int n) {
+    memset(R, 0, sizeof *R);
+    R->m = m; R->n = n;
+    nnls_ws_init(&R->plain, m, n);
+    nnls_ws_init(&R->tikh, m + n, n);
+    R->cx = calloc((size_t)TIKH_CACHE_N * n, sizeof(double));
+    R->cidx = calloc((size_t)TIKH_CACHE_N * n, sizeof(int));
+    R->gamma = calloc((size_t)(m < n ? m : n), sizeof(double));
+    for (int i = 0; i < TIKH_CACHE_N; ++i) R->cmu[i] = NAN;
+    R->ccur = 0;
+}
+static void reg_ws_free(reg_ws *R) {
+    nnls_ws_free(&R->plain); nnls_ws_free(&R->tikh);
+    free(R->cx); free(R->cidx); free(R->gamma);
+}
+static void reg_bind(reg_ws *R, const double *A, const double *b) { R->A = A; R->b = b; }
+static void cache_reset(reg_ws *R) { for (int i = 0; i < TIKH_CACHE_N; ++i) R->cmu[i] = NAN; }
+
+/* solve!(::NNLSTikhonovRegProblemCache, μ) (:417-444) */
+static const double *cache_solve(reg_ws *R, double mu) {
+    int hit = -1;
+    for (int i = 0; i < TIKH_CACHE_N; ++i)
+        if (!isnan(R->cmu[i]) && R->cmu[i] == mu) { hit = i; break; }
+    if (hit >= 0) { R->ccur = hit; return R->cx + (size_t)hit * R->n; }
+    /* next_cache_index! (:404-413) */
+    int slot = -1;
+    for (int i = 0; i < TIKH_CACHE_N; ++i)
+        if (isnan(R->cmu[i])) { slot = i; break; }
+    if (slot < 0) slot = (R->ccur + 1) % TIKH_CACHE_N;
+    R->ccur = slot;
+    nnlsprob_solve(&R->tikh, R->A, R->b, R->m, R->n, mu, -1);
+    R->nsolves++;
+    R->cmu[slot] = mu;
+    R->crnorm[slot] = R->tikh.rnorm;
+    R->cnsetp[slot] = R->tikh.nsetp;
+    memcpy(R->cx + (size_t)slot * R->n, R->tikh.x, sizeof(double) * R->n);
+    memcpy(R->cidx + (size_t)slot * R->n, R->tikh.idx, sizeof(int) * R->n);
+    return R->cx + (size_t)slot * R->n;
+}
+/* seminorm_sq / loss / regnorm / resnorm_sq on the current entry (:307-320) */
+static double cur_seminorm_sq(const reg_ws *R) {
+    const double *x = R->cx + (size_t)R->ccur * R->n;
+    const int *idx = R->cidx + (size_t)R->ccur * R->n;
+    double s = 0.0;
+    for (int i = 0; i < R->cnsetp[R->ccur]; ++i) { double v = x[idx[i] - 1]; s += v * v; }
+    return s;
+}
+static double cur_resnorm_sq(const reg_ws *R) {
+    double mu = R->cmu[R->ccur], rn = R->crnorm[R->ccur];
+    double loss = rn * rn;
+    double regnorm = (mu * mu) * cur_seminorm_sq(R);
+    return jl_max(loss - regnorm, 0.0);
+}
+static const double *plain_solve(reg_ws *R) {
+    nnlsprob_solve(&R->plain, R->A, R->b, R->m, R->n, -1.0, -1);
+    R->nsolves++;
+    return R->plain.x;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Scalar root finding and minimization (src/optimization.jl)
+ * ------------------------------------------------------------------------------------------------ */
+typedef double (*fun1_t)(double, void *);
+
+/* bracket_root_monotonic (:191-219) */
+static void bracket_root_monotonic(fun1_t f, void *ctx, double a, double delta, double dilate, double mono,
+                                   int maxiters, double out[4]) {
+    double fa = f(a, ctx);
+    if (!isfinite(fa)) { out[0] = a; out[1] = a; out[2] = NAN; out[3] = NAN; return; }
+    if (fa == 0) { out[0] = a; out[1] = a; out[2] = fa; out[3] = fa; return; }
+    double sgn = jl_sign(mono) * jl_sign(fa);
+    double b = a - sgn * delta;
+    double fb = f(b, ctx);
+    if (!isfinite(fb)) { out[0] = a; out[1] = a; out[2] = fa; out[3] = fa; return; }
+    if (fb == 0) { out[0] = b; out[1] = b; out[2] = fb; out[3] = fb; return; }
+    delta *= dilate;
+    int cnt = 0;
+    while (fa * fb > 0 && cnt < maxiters) {
+        a = b; fa = fb;
+        b = a - sgn * delta;
+        fb = f(b, ctx);
+        if (!isfinite(fb)) { out[0] = a; out[1] = a; out[2] = fa; out[3] = fa; return; }
+        if (fb == 0) { out[0] = b; out[1] = b; out[2] = fb; out[3] = fb; return; }
+        delta *= dilate;
+        cnt += 1;
+    }
+    if (a < b) { out[0] = a; out[1] = b; out[2] = fa; out[3] = fb; }
+    else { out[0] = b; out[1] = a; out[2] = fb; out[3] = fa; }
+}
+
+/* brent_root (:71-128) with secant_step/inverse_quadratic_step (:177-189) */
+static void brent_root(fun1_t f, void *ctx, double x0, double x1, double fx0, double fx1, double xatol, double xrtol,
+                       double ftol, int maxiters, double *xo, double *fo) {
+    if (fx0 == 0) { *xo = x0; *fo = fx0; return; }
+    if (fx1 == 0) { *xo = x1; *fo = fx1; return; }
+    double a = x0, b = x1, fa = fx0, fb = fx1;
+    if (fabs(fa) < fabs(fb)) { double t = a; a = b; b = t; t = fa; fa = fb; fb = t; }
+    double c = x0, d = x0, fc = fx0;
+    int mflag = 1;
+    for (int iter = 1; iter <= maxiters; ++iter) {
+        if (fabs(b - a) <= 2 * (xatol + xrtol * fabs(b))) { *xo = b; *fo = fb; return; }
+        double s = 0.0;
+        s += a * fb * fc / (fa - fb) / (fa - fc);
+        s += b * fa * fc / (fb - fa) / (fb - fc);
+        s += c * fa * fb / (fc - fa) / (fc - fb);
+        if (isnan(s) || isinf(s)) s = a - fa * (b - a) / (fb - fa);
+        double u = (3 * a + b) / 4, v = b;
+        if (u > v) { double t = u; u = v; v = t; }
+        double tol = jl_max(xatol, xrtol * jl_max(jl_max(fabs(b), fabs(c)), fabs(d)));
+        if (!(u < s && s < v) || (mflag && fabs(s - b) >= fabs(b - c) / 2) || (!mflag && fabs(s - b) >= fabs(b - c) / 2) ||
+            (mflag && fabs(b - c) <= tol) || (!mflag && fabs(c - d) <= tol)) {
+            s = (a + b) / 2;
+            mflag = 1;
+        } else {
+            mflag = 0;
+        }
+        double fs = f(s, ctx);
+        if (fs == 0) { *xo = s; *fo = fs; return; }
+        if (isnan(fs) || isinf(fs)) { *xo = b; *fo = fb; return; }
+        if (fabs(fs) <= ftol) { *xo = s; *fo = fs; return; }
+        /* `c, fc, d = b, fb, c`: d takes the old c */
+        d = c; c = b; fc = fb;
+        if (jl_sign(fa) * jl_sign(fs) < 0) { b = s; fb = fs; }
+        else { a = s; fa = fs; }
+        if (fabs(fa) < fabs(fb)) { double t = a; a = b; b = t; t = fa; fa = fb; fb = t; }
+    }
+    *xo = b; *fo = fb;
+}
+
+/* brent_minimize (:319-413) */
+static void brent_minimize(fun1_t f, void *ctx, double x1, double x2, double xrtol, double xatol, int maxiters,
+                           double *xo, double *yo) {
+    const double alpha = 2 - GOLDEN;
+    double x = x1 + alpha * (x2 - x1);
+    double y = f(x, ctx);
+    double dx_old = 0.0, dx = 0.0;
+    double x_older = x, x_old = x, y_older = y, y_old = y;
+    int iter = 0;
+    while (iter < maxiters) {
+        double p = 0.0, q = 0.0;
+        double xm = (x2 + x1) / 2;
+        double dx_tol = xatol + xrtol * fabs(x);
+        if (fabs(x - xm) + (x2 - x1) / 2 <= 2 * dx_tol) break;
+        iter += 1;
+        if (fabs(dx_old) > dx_tol) {
+            double r = (x - x_old) * (y - y_older);
+            q = (x - x_older) * (y - y_old);
+            p = (x - x_older) * q - (x - x_old) * r;
+            q = 2 * (q - r);
+            if (q > 0) p = -p;
+            else q = -q;
+        }
+        if (fabs(p) < fabs(q * dx_old / 2) && p < q * (x2 - x) && p < q * (x - x1)) {
+            dx_old = dx;
+            dx = p / q;
+            double x_tmp = x + dx;
+            if ((x_tmp - x1) < 2 * dx_tol || (x2 - x_tmp) < 2 * dx_tol) dx = (x < xm) ? dx_tol : -dx_tol;
+        } else {
+            dx_old = (x < xm) ? x2 - x : x1 - x;
+            dx = alpha * dx_old;
+        }
+        double x_new = (fabs(dx) >= dx_tol) ? x + dx : x + ((dx > 0) ? dx_tol : -dx_tol);
+        double y_new = f(x_new, ctx);
+        if (y_new < y) {
+            if (x_new < x) x2 = x;
+            else x1 = x;
+            x_older = x_old; x_old = x; x = x_new;
+            y_older = y_old; y_old = y; y = y_new;
+        } else {
+            if (x_new < x) x1 = x_new;
+            else x2 = x_new;
+            if (y_new <= y_old || x_old == x) {
+                x_older = x_old; x_old = x_new;
+                y_older = y_old; y_old = y_new;
+            } else if (y_new <= y_older || x_older == x || x_older == x_old) {
+                x_older = x_new;
+                y_older = y_new;
+            }
+        }
+    }
+    *xo = x; *yo = y;
+}
+
+void orc_brent_root(orc_fun1 f, void *ctx, double x0, double x1, double xatol, double xrtol, double ftol, int maxiters,
+                    double *xout, double *fout) {
+    brent_root(f, ctx, x0, x1, f(x0, ctx), f(x1, ctx), xatol, xrtol, ftol, maxiters, xout, fout);
+}
+void orc_bracket_root_monotonic(orc_fun1 f, void *ctx, double a, double delta, double dilate, double mono, int maxiters,
+                                double out[4]) {
+    bracket_root_monotonic(f, ctx, a, delta, dilate, mono, maxiters, out);
+}
+void orc_brent_minimize(orc_fun1 f, void *ctx, double x1, double x2, double xrtol, double xatol, int maxiters,
+                        double *xout, double *fout) {
+    brent_minimize(f, ctx, x1, x2, xrtol, xatol, maxiters, xout, fout);
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * chi2 / mdp / none (src/lsqnonneg.jl:504-593, 700-747; src/T2mapSEcorr.jl:475-479)
+ * ------------------------------------------------------------------------------------------------ */
+typedef struct { reg_ws *R; double target; } chi2_ctx;
+static double chi2_relerr_f(double logmu, void *p) {
+    chi2_ctx *c = p;
+    cache_solve(c->R, exp(logmu));
+    double res2 = cur_resnorm_sq(c->R);
+    return (res2 - c->target) / c->target;
+}
+static void lsq_chi2(reg_ws *R, double chi2_target, double *x_out, double *mu_out, double *chi2_out) {
+    const double *xu = plain_solve(R);
+    double res2_min = R->plain.rnorm * R->plain.rnorm;
+    int n = R->n;
+    if (res2_min == 0 || R->plain.nsetp == 0) {
+        memcpy(x_out, xu, sizeof(double) * n);
+        *mu_out = 0; *chi2_out = 1;
+        return;
+    }
+    double target = chi2_target * res2_min;
+    cache_reset(R);
+    chi2_ctx c = {R, target};
+    double br[4];
+    bracket_root_monotonic(chi2_relerr_f, &c, -4.0, 1.0, 1.5, 1.0, 6, br);
+    double a = br[0], b = br[1], fa = br[2], fb = br[3], xf, ff;
+    if (fa * fb < 0)
+        brent_root(chi2_relerr_f, &c, a, b, fa, fb, 0.0, 0.0, 1e-3 * (chi2_target - 1), 100, &xf, &ff);
+    else if (!isfinite(fa)) { xf = b; ff = fb; }
+    else if (!isfinite(fb)) { xf = a; ff = fa; }
+    else if (fabs(fa) < fabs(fb)) { xf = a; ff = fa; }
+    else { xf = b; ff = fb; }
+    double mu_final, res2_final;
+    if (isfinite(ff)) {
+        mu_final = exp(xf);
+        res2_final = target * (1 + ff);
+        memcpy(x_out, cache_solve(R, mu_final), sizeof(double) * n);
+    } else {
+        memcpy(x_out, xu, sizeof(double) * n);
+        mu_final = 0; res2_final = 1;
+    }
+    *mu_out = mu_final;
+    *chi2_out = res2_final / res2_min;
+}
+
+typedef struct { reg_ws *R; double delta2; } mdp_ctx;
+static double mdp_f(double logmu, void *p) {
+    mdp_ctx *c = p;
+    cache_solve(c->R, exp(logmu));
+    return cur_resnorm_sq(c->R) - c->delta2;
+}
+static void lsq_mdp(reg_ws *R, double delta, double *x_out, double *mu_out, double *chi2_out) {
+    const double *xu = plain_solve(R);
+    int n = R->n, m = R->m;
+    double res2_min = R->plain.rnorm * R->plain.rnorm;
+    if (delta <= sqrt(res2_min)) {
+        memcpy(x_out, xu, sizeof(double) * n);
+        *mu_out = 0; *chi2_out = 1;
+        return;
+    }
+    double res2_max = 0.0;
+    for (int i = 0; i < m; ++i) res2_max += R->b[i] * R->b[i];
+    if (delta >= sqrt(res2_max)) {
+        for (int j = 0; j < n; ++j) x_out[j] = 0.0;
+        *mu_out = INFINITY;
+        *chi2_out = res2_max / res2_min;
+        return;
+    }
+    cache_reset(R);
+    mdp_ctx c = {R, delta * delta};
+    double br[4];
+    bracket_root_monotonic(mdp_f, &c, -4.0, 1.0, 1.5, 1.0, 6, br);
+    double a = br[0], b = br[1], fa = br[2], fb = br[3], xf, ff;
+    if (fa * fb < 0)
+        brent_root(mdp_f, &c, a, b, fa, fb, 0.0, 0.0, 1e-3 * (delta * delta), 100, &xf, &ff);
+    else if (!isfinite(fa)) { xf = b; ff = fb; }
+    else if (!isfinite(fb)) { xf = a; ff = fa; }
+    else if (fabs(fa) < fabs(fb)) { xf = a; ff = fa; }
+    else { xf = b; ff = fb; }
+    double mu_final, res2_final;
+    if (isfinite(ff)) {
+        mu_final = exp(xf);
+        res2_final = delta * delta + ff;
+        memcpy(x_out, cache_solve(R, mu_final), sizeof(double) * n);
+    } else {
+        memcpy(x_out, xu, sizeof(double) * n);
+        mu_final = 0; res2_final = 1;
+    }
+    *mu_out = mu_final;
+    *chi2_out = res2_final / res2_min;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * GCV (src/lsqnonneg.jl:1136-1342). Singular values by one-sided Jacobi (Hestenes); the reference calls
+ * LAPACK dgesdd through libblastrampoline (src/utils.jl:96-134).
+ * ------------------------------------------------------------------------------------------------ */
+static int cmp_desc(const void *a, const void *b) {
+    double x = *(const double *)a, y = *(const double *)b;
+    return (x < y) - (x > y);
+}
+void orc_svdvals(const double *A, int m, int n, double *s) {
+    int tr = m < n;
+    int r = tr ? n : m, c = tr ? m : n; /* work on an r x c matrix with r >= c */
+    double *U = malloc(sizeof(double) * (size_t)r * c);
+    for (int j = 0; j < c; ++j)
+        for (int i = 0; i < r; ++i) U[i + (size_t)j * r] = tr ? A[j + (size_t)i * m] : A[i + (size_t)j * m];
+    for (int sweep = 0; sweep < 80; ++sweep) {
+        double off = 0.0;
+        for (int p = 0; p < c - 1; ++p)
+            for (int q = p + 1; q < c; ++q) {
+                double al = 0, be = 0, ga = 0;
+                double *up = U + (size_t)p * r, *uq = U + (size_t)q * r;
+                _Pragma("omp simd reduction(+:al,be,ga)")
+                for (int i = 0; i < r; ++i) { al += up[i] * up[i]; be += uq[i] * uq[i]; ga += up[i] * uq[i]; }
+                if (ga == 0.0 || fabs(ga) <= EPS64 * sqrt(al * be)) continue;
+                off = fmax(off, fabs(ga) / sqrt(al * be));
+                double zeta = (be - al) / (2 * ga);
+                double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+                double cs = 1 / sqrt(1 + t * t), sn = cs * t;
+                for (int i = 0; i < r; ++i) {
+                    double x = up[i], y = uq[i];
+                    up[i] = cs * x - sn * y;
+                    uq[i] = sn * x + cs * y;
+                }
+            }
+        if (off <= EPS64) break;
+        if (getenv("ORC_SVD_DEBUG")) fprintf(stderr, "sweep %d off %g\n", sweep, off);
+    }
+    for (int j = 0; j < c; ++j) {
+        double ss = 0;
+        for (int i = 0; i < r; ++i) ss += U[i + (size_t)j * r] * U[i + (size_t)j * r];
+        s[j] = sqrt(ss);
+    }
+    qsort(s, (size_t)c, sizeof(double), cmp_desc);
+    free(U);
+}
+
+/* gcv_dof (:1321-1329) */
+double orc_gcv_dof(int m, int n, const double *gamma, int ngamma, double lambda) {
+    double dof = (double)(m - n > 0 ? m - n : 0);
+    double l2 = lambda * lambda;
+    for (int i = 0; i < ngamma; ++i) {
+        double g2 = gamma[i] * gamma[i];
+        dof += l2 / (g2 + l2);
+    }
+    return dof;
+}
+
+typedef struct { reg_ws *R; double low; } gcv_ctx;
+static double loggcv_f(double logmu, void *p) {
+    gcv_ctx *c = p;
+    double mu = exp(logmu);
+    cache_solve(c->R, mu);
+    double res2 = cur_resnorm_sq(c->R);
+    double dof = orc_gcv_dof(c->R->m, c->R->n, c->R->gamma, c->R->ngamma, mu);
+    double g = res2 / (dof * dof);
+    g = jl_max(g, c->low);
+    return log(g);
+}
+static void lsq_gcv(reg_ws *R, double *x_out, double *mu_out, double *chi2_out) {
+    int m = R->m, n = R->n;
+    R->ngamma = m < n ? m : n;
+    orc_svdvals(R->A, m, n, R->gamma);
+    gcv_ctx c = {R, (EPS64 * EPS64) / m};
+    cache_reset(R);
+    double xf, yf;
+    brent_minimize(loggcv_f, &c, -8.0, 2.0, 0.0, 1e-4, 20, &xf, &yf);
+    double mu_final = exp(xf);
+    memcpy(x_out, cache_solve(R, mu_final), sizeof(double) * n);
+    double res2 = cur_resnorm_sq(R);
+    plain_solve(R);
+    double res2u = R->plain.rnorm * R->plain.rnorm;
+    *mu_out = mu_final;
+    *chi2_out = res2 / res2u;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * L-curve (src/lsqnonneg.jl:812-972) with GrowableCache/CachedFunction/mapfindmax semantics
+ * (src/utils.jl:140-254): isapprox-keyed caches, first-maximum findmax with NaN as largest.
+ * ------------------------------------------------------------------------------------------------ */
+typedef struct { double x; double P[2]; } fpt;
+typedef struct { double x; double P[2]; double C; } cpt;
+typedef struct { double x[4]; double P[4][2]; } lstate;
+typedef struct {
+    orc_fun2 f; void *ctx;
+    fpt *fc; int nfc, capfc;
+    cpt *pc; int npc, cappc;
+    lstate *sc; int nsc, capsc;
+} lc_t;
+
+static void lc_eval(lc_t *L, double x, double out[2]) {
+    for (int i = 0; i < L->nfc; ++i)
+        if (jl_isapprox(x, L->fc[i].x)) { out[0] = L->fc[i].P[0]; out[1] = L->fc[i].P[1]; return; }
+    L->f(x, L->ctx, out);
+    if (L->nfc == L->capfc) { L->capfc = L->capfc ? 2 * L->capfc : 64; L->fc = realloc(L->fc, sizeof(fpt) * L->capfc); }
+    L->fc[L->nfc].x = x; L->fc[L->nfc].P[0] = out[0]; L->fc[L->nfc].P[1] = out[1];
+    L->nfc++;
+}
+static void pc_push(lc_t *L, double x, const double P[2], double C) {
+    if (L->npc == L->cappc) { L->cappc = L->cappc ? 2 * L->cappc : 64; L->pc = realloc(L->pc, sizeof(cpt) * L->cappc); }
+    L->pc[L->npc].x = x; L->pc[L->npc].P[0] = P[0]; L->pc[L->npc].P[1] = P[1]; L->pc[L->npc].C = C;
+    L->npc++;
+}
+static int pc_find(const lc_t *L, double x) {
+    for (int i = 0; i < L->npc; ++i)
+        if (jl_isapprox(x, L->pc[i].x)) return i;
+    return -1;
+}
+static double menger(const double Pj[2], const double Pk[2], const double Pl[2]) {
+    double djk0 = Pj[0] - Pk[0], djk1 = Pj[1] - Pk[1];
+    double dkl0 = Pk[0] - Pl[0], dkl1 = Pk[1] - Pl[1];
+    double dlj0 = Pl[0] - Pj[0], dlj1 = Pl[1] - Pj[1];
+    double PjPk = djk0 * djk0 + djk1 * djk1, PkPl = dkl0 * dkl0 + dkl1 * dkl1, PlPj = dlj0 * dlj0 + dlj1 * dlj1;
+    double cross = djk0 * dkl1 - djk1 * dkl0;
+    return 2 * cross / sqrt(PjPk * PkPl * PlPj);
+}
+static void update_curvature(lc_t *L, const lstate *s, const double Ptl[2], const double Pbr[2], double Ctol) {
+    for (int i = 0; i < 4; ++i) {
+        double x = s->x[i], C = -INFINITY;
+        const double *P = s->P[i];
+        double d1 = norm2(P[0] - Ptl[0], P[1] - Ptl[1]), d2 = norm2(P[0] - Pbr[0], P[1] - Pbr[1]);
+        if (jl_min(d1, d2) > Ctol) {
+            double xm = -INFINITY, xp = INFINITY;
+            const double *Pm = P, *Pp = P;
+            for (int k = 0; k < L->npc; ++k) {
+                double xk = L->pc[k].x;
+                if (xm < xk && xk < x) { xm = xk; Pm = L->pc[k].P; }
+                if (x < xk && xk < xp) { xp = xk; Pp = L->pc[k].P; }
+            }
+            double Pmc[2] = {Pm[0], Pm[1]}, Ppc[2] = {Pp[0], Pp[1]};
+            C = menger(Pmc, P, Ppc);
+        }
+        int ind = pc_find(L, x);
+        if (ind < 0) pc_push(L, x, P, C);
+        else { L->pc[ind].P[0] = P[0]; L->pc[ind].P[1] = P[1]; L->pc[ind].C = C; }
+    }
+}
+static int pc_argmax(const lc_t *L) {
+    int best = 0;
+    for (int i = 1; i < L->npc; ++i)
+        if (jl_isless(L->pc[best].C, L->pc[i].C)) best = i;
+    return best;
+}
+static atomic_int g_lc_max_fc, g_lc_max_pc, g_lc_max_sc;
+static void atomic_max_int(atomic_int *a, int v) {
+    int cur = atomic_load(a);
+    while (v > cur && !atomic_compare_exchange_weak(a, &cur, v)) {}
+}
+/* Largest L-curve cache sizes seen since the last call (sizing data for fixed-capacity implementations). */
+void orc_lcurve_cache_stats(int out[3]) {
+    out[0] = atomic_exchange(&g_lc_max_fc, 0);
+    out[1] = atomic_exchange(&g_lc_max_pc, 0);
+    out[2] = atomic_exchange(&g_lc_max_sc, 0);
+}
+double orc_lcurve_corner(orc_fun2 f, void *ctx, double xlow, double xhigh, double xtol, double Ptol, double Ctol) {
+    lc_t L;
+    memset(&L, 0, sizeof L);
+    L.f = f; L.ctx = ctx;
+    lstate st;
+    /* initial_state (:921-929) */
+    double x1 = xlow, x4 = xhigh;
+    double x2 = (GOLDEN * x1 + x4) / (GOLDEN + 1);
+    double x3 = x1 + (x4 - x2);
+    st.x[0] = x1; st.x[1] = x2; st.x[2] = x3; st.x[3] = x4;
+    for (int i = 0; i < 4; ++i) lc_eval(&L, st.x[i], st.P[i]);
+    for (int i = 0; i < 4; ++i) pc_push(&L, st.x[i], st.P[i], -INFINITY);
+    double Ptl[2] = {st.P[0][0], st.P[0][1]}, Pbr[2] = {st.P[3][0], st.P[3][1]};
+    update_curvature(&L, &st, Ptl, Pbr, Ctol);
+    int iter = 0;
+    while (!(fabs(st.x[3] - st.x[0]) < xtol || norm2(st.P[0][0] - st.P[3][0], st.P[0][1] - st.P[3][1]) < Ptol)) {
+        iter += 1;
+        /* backtracking (:892-900) */
+        double xb = L.pc[pc_argmax(&L)].x;
+        for (int k = 0; k < L.nsc; ++k) {
+            const lstate *s = &L.sc[k];
+            if ((s->x[1] == xb || s->x[2] == xb) && fabs(s->x[3] - s->x[0]) <= fabs(st.x[3] - st.x[0])) st = *s;
+        }
+        int i2 = pc_find(&L, st.x[1]), i3 = pc_find(&L, st.x[2]);
+        if (L.pc[i2].C > L.pc[i3].C) {
+            /* move_left (:933-939) */
+            lstate ns;
+            ns.x[0] = st.x[0]; ns.x[1] = (GOLDEN * st.x[0] + st.x[2]) / (GOLDEN + 1); ns.x[2] = st.x[1]; ns.x[3] = st.x[2];
+            memcpy(ns.P[0], st.P[0], sizeof(double) * 2);
+            lc_eval(&L, ns.x[1], ns.P[1]);
+            memcpy(ns.P[2], st.P[1], sizeof(double) * 2);
+            memcpy(ns.P[3], st.P[2], sizeof(double) * 2);
+            st = ns;
+        } else {
+            /* move_right (:941-946) */
+            lstate ns;
+            ns.x[0] = st.x[1]; ns.x[1] = st.x[2]; ns.x[2] = st.x[1] + (st.x[3] - st.x[2]); ns.x[3] = st.x[3];
+            memcpy(ns.P[0], st.P[1], sizeof(double) * 2);
+            memcpy(ns.P[1], st.P[2], sizeof(double) * 2);
+            lc_eval(&L, ns.x[2], ns.P[2]);
+            memcpy(ns.P[3], st.P[3], sizeof(double) * 2);
+            st = ns;
+        }
+        update_curvature(&L, &st, Ptl, Pbr, Ctol);
+        if (L.nsc == L.capsc) { L.capsc = L.capsc ? 2 * L.capsc : 64; L.sc = realloc(L.sc, sizeof(lstate) * L.capsc); }
+        L.sc[L.nsc++] = st;
+    }
+    double xbest = L.pc[pc_argmax(&L)].x;
+    atomic_max_int(&g_lc_max_fc, L.nfc);
+    atomic_max_int(&g_lc_max_pc, L.npc);
+    atomic_max_int(&g_lc_max_sc, L.nsc);
+    free(L.fc); free(L.pc); free(L.sc);
+    return xbest;
+}
+
+static void lcurve_point(double logmu, void *p, double out[2]) {
+    reg_ws *R = p;
+    cache_solve(R, exp(logmu));
+    out[0] = log(cur_resnorm_sq(R));
+    out[1] = log(cur_seminorm_sq(R));
+}
+static void lsq_lcurve(reg_ws *R, double *x_out, double *mu_out, double *chi2_out) {
+    int n = R->n;
+    cache_reset(R);
+    double logmu = orc_lcurve_corner(lcurve_point, R, -8.0, 2.0, 1e-4, 1e-4, 1e-4);
+    double mu_final = exp(logmu);
+    memcpy(x_out, cache_solve(R, mu_final), sizeof(double) * n);
+    double res2 = cur_resnorm_sq(R);
+    plain_solve(R);
+    double res2u = R->plain.rnorm * R->plain.rnorm;
+    *mu_out = mu_final;
+    *chi2_out = res2 / res2u;
+}
+
+static void lsq_none(reg_ws *R, double *x_out, double *mu_out, double *chi2_out) {
+    memcpy(x_out, plain_solve(R), sizeof(double) * R->n);
+    *mu_out = 0; *chi2_out = 1;
+}
+
+#define REG_API(NAME, CALL)                                                                                      \
+    void NAME {                                                                                                  \
+        reg_ws R;                                                                                                \
+        reg_ws_init(&R, m, n);                                                                                   \
+        reg_bind(&R, A, b);                                                                                      \
+        CALL;                                                                                                    \
+        reg_ws_free(&R);                                                                                         \
+    }
+REG_API(orc_lsqnonneg_none(const double *A, const double *b, int m, int n, double *x, double *mu, double *chi2),
+        lsq_none(&R, x, mu, chi2))
+REG_API(orc_lsqnonneg_chi2(const double *A, const double *b, int m, int n, double chi2_target, double *x, double *mu,
+                           double *chi2),
+        lsq_chi2(&R, chi2_target, x, mu, chi2))
+REG_API(orc_lsqnonneg_mdp(const double *A, const double *b, int m, int n, double delta, double *x, double *mu,
+                          double *chi2),
+        lsq_mdp(&R, delta, x, mu, chi2))
+REG_API(orc_lsqnonneg_gcv(const double *A, const double *b, int m, int n, double *x, double *mu, double *chi2),
+        lsq_gcv(&R, x, mu, chi2))
+REG_API(orc_lsqnonneg_lcurve(const double *A, const double *b, int m, int n, double *x, double *mu, double *chi2),
+        lsq_lcurve(&R, x, mu, chi2))
+
+/* ------------------------------------------------------------------------------------------------
+ * Flip-angle search: cubic Hermite surrogate + discrete bisection (src/splines.jl:53-110, 504-566,
+ * 664-850, 975-1051) driven by optimize_flip_angle! (src/T2mapSEcorr.jl:409-423).
+ * ------------------------------------------------------------------------------------------------ */
+static inline double evalpoly4(double t, const double c[4]) { return fma(t, fma(t, fma(t, c[3], c[2]), c[1]), c[0]); }
+
+static void hermite_minimize(double a, double b, double u0, double u1, double m0, double m1, double out[2]) {
+    double r = (b - a) / 2;
+    m0 = r * m0; m1 = r * m1;
+    double du = u1 - u0, dm = m1 - m0, su = u1 + u0, sm = m1 + m0;
+    double coeffs[4] = {su / 2 - dm / 4, (3 * du - sm) / 4, dm / 4, (sm - du) / 4};
+    double xend = u0 < u1 ? a : b, uend = u0 < u1 ? u0 : u1;
+    double D = 3 * (u0 - u1);
+    double th = D / 2 + (m0 + m1);
+    double g = th * th - m0 * m1;
+    g = g > 0 ? -sqrt(g) : 0.0;
+    double p = -(D + (m0 + m1));
+    double q = 2 * g + (m0 - m1);
+    if (fabs(p) < fabs(q)) {
+        double t = p / q;
+        double y = evalpoly4(t, coeffs);
+        if (y < uend) {
+            double c = (a + b) / 2, rr = (b - a) / 2;
+            double x = fma(rr, t, c);
+            x = x < a ? a : (x > b ? b : x);
+            out[0] = x; out[1] = y;
+            return;
+        }
+    }
+    out[0] = xend; out[1] = uend;
+}
+void orc_hermite_minimize(double a, double b, double u0, double u1, double m0, double m1, double out[2]) {
+    hermite_minimize(a, b, u0, u1, m0, m1, out);
+}
+
+typedef struct {
+    int m, n, K;
+    const double *As, *dAs, *alphas, *b;
+    nnls_ws nw;
+    double *Axb, *dAx;
+    /* searcher + surrogate state */
+    unsigned char *seen;
+    double *u, *du;
+    int *sidx;
+    int npts, numeval, mineval, maxeval;
+} flip_ws;
+
+static void flip_ws_init(flip_ws *F, int m, int n, int K) {
+    memset(F, 0, sizeof *F);
+    F->m = m; F->n = n; F->K = K;
+    nnls_ws_init(&F->nw, m, n);
+    F->Axb = calloc((size_t)m, sizeof(double));
+    F->dAx = calloc((size_t)m, sizeof(double));
+    F->seen = calloc((size_t)K + 1, 1);
+    F->u = calloc((size_t)K + 1, sizeof(double));
+    F->du = calloc((size_t)K + 1, sizeof(double));
+    F->sidx = calloc((size_t)K + 1, sizeof(int));
+}
+static void flip_ws_free(flip_ws *F) {
+    nnls_ws_free(&F->nw);
+    free(F->Axb); free(F->dAx); free(F->seen); free(F->u); free(F->du); free(F->sidx);
+}
+
+/* loss_with_grad! (src/splines.jl:1010-1041) at grid index I (1-based) */
+static void flip_eval(flip_ws *F, int I) {
+    int m = F->m, n = F->n;
+    const double *A = F->As + (size_t)(I - 1) * m * n, *dA = F->dAs + (size_t)(I - 1) * m * n;
+    nnlsprob_solve(&F->nw, A, F->b, m, n, -1.0, -1);
+    double ell = F->nw.rnorm * F->nw.rnorm;
+    const double *x = F->nw.x;
+    for (int i = 0; i < m; ++i) F->Axb[i] = 0.0;
+    for (int j = 0; j < n; ++j)
+        if (x[j] > 0)
+            for (int i = 0; i < m; ++i) F->Axb[i] = fma(x[j], A[i + (size_t)j * m], F->Axb[i]);
+    for (int i = 0; i < m; ++i) F->Axb[i] -= F->b[i];
+    for (int i = 0; i < m; ++i) F->dAx[i] = 0.0;
+    for (int j = 0; j < n; ++j)
+        if (x[j] > 0)
+            for (int i = 0; i < m; ++i) F->dAx[i] = fma(x[j], dA[i + (size_t)j * m], F->dAx[i]);
+    double dt = 0.0;
+    for (int i = 0; i < m; ++i) dt += F->dAx[i] * F->Axb[i];
+    F->u[I] = ell;
+    F->du[I] = 2 * dt;
+}
+
+/* update!(surr, state, I; maxeval) (:736-744) + CubicHermiteSplineSurrogate update! (:526-534) */
+static int flip_update(flip_ws *F, int I) {
+    if (F->numeval >= F->maxeval) return 1;
+    if (F->seen[I]) return 0;
+    flip_eval(F, I);
+    /* insertsorted! (src/utils.jl:42-49) */
+    int len = ++F->npts, val = I;
+    for (int i = 1; i <= len - 1; ++i) {
+        int xi = F->sidx[i];
+        int lo = xi < val ? xi : val, hi = xi < val ? val : xi;
+        F->sidx[i] = lo; val = hi;
+    }
+    F->sidx[len] = val;
+    F->seen[I] = 1;
+    F->numeval += 1;
+    return F->numeval >= F->maxeval;
+}
+static int box_evaluated(const flip_ws *F, int lo, int hi) { return (F->seen[lo] + F->seen[hi]) >= 2; }
+
+/* evaluate_box! (:802-815); x NaN = natural corner order, else sorted by distance to x */
+static void evaluate_box(flip_ws *F, int lo, int hi, double x) {
+    int cs[2] = {lo, hi};
+    if (!isnan(x)) {
+        double d0 = (F->alphas[lo - 1] - x) * (F->alphas[lo - 1] - x), d1 = (F->alphas[hi - 1] - x) * (F->alphas[hi - 1] - x);
+        if (d1 < d0) { cs[0] = hi; cs[1] = lo; }
+    }
+    for (int k = 0; k < 2; ++k) {
+        if (box_evaluated(F, lo, hi)) break;
+        if (flip_update(F, cs[k])) break;
+    }
+}
+/* initialize! (:716-734) */
+static void flip_init_rec(flip_ws *F, int lo, int hi, int depth) {
+    if (depth <= 0) return;
+    evaluate_box(F, lo, hi, NAN);
+    if (F->numeval >= F->mineval) return;
+    int mid = (lo + hi) / 2;
+    flip_init_rec(F, lo, mid, depth - 1);
+    flip_init_rec(F, mid, hi, depth - 1);
+}
+/* suggest_point (:544-566) */
+static void suggest_point(const flip_ws *F, double *xo, double *uo) {
+    int I0 = F->sidx[1];
+    double plast = F->alphas[I0 - 1], ulast = F->u[I0], dlast = F->du[I0];
+    double p = plast, u = ulast;
+    for (int i = 2; i <= F->npts; ++i) {
+        int I = F->sidx[i];
+        double pc = F->alphas[I - 1], uc = F->u[I], dc = F->du[I];
+        double r[2];
+        hermite_minimize(plast, pc, ulast, uc, dlast, dc, r);
+        if (r[1] < u) { p = r[0]; u = r[1]; }
+        plast = pc; ulast = uc; dlast = dc;
+    }
+    *xo = p; *uo = u;
+}
+
+static double flip_search(flip_ws *F, const double *b, int mineval, int *nevals) {
+    int K = F->K;
+    F->b = b;
+    memset(F->seen, 0, (size_t)K + 1);
+    F->npts = 0; F->numeval = 0;
+    F->mineval = mineval; F->maxeval = K;
+    for (int depth = 1; depth <= mineval; ++depth) {
+        flip_init_rec(F, 1, K, depth);
+        if (F->numeval >= mineval) break;
+    }
+    /* bisection_search (:750-775) with minimal_bounding_box (:778-800) */
+    double x, u;
+    suggest_point(F, &x, &u);
+    for (;;) {
+        int lo = 1, hi = K, blo, bhi;
+        for (;;) {
+            int mid = (lo + hi) / 2;
+            if (F->alphas[lo - 1] <= x && x <= F->alphas[mid - 1]) {
+                if (!box_evaluated(F, lo, mid) || !(mid - lo > 1)) { blo = lo; bhi = mid; break; }
+                hi = mid;
+            } else {
+                if (!box_evaluated(F, mid, hi) || !(hi - mid > 1)) { blo = mid; bhi = hi; break; }
+                lo = mid;
+            }
+        }
+        evaluate_box(F, blo, bhi, x);
+        suggest_point(F, &x, &u);
+        if (F->numeval >= F->maxeval || (bhi - blo) <= 1) break;
+    }
+    if (nevals) *nevals = F->numeval;
+    return x;
+}
+
+double orc_flip_search(const double *As, const double *dAs, const double *alphas, int m, int n, int K, int mineval,
+                       const double *b, int *nevals) {
+    flip_ws F;
+    flip_ws_init(&F, m, n, K);
+    F.As = As; F.dAs = dAs; F.alphas = alphas;
+    double a = flip_search(&F, b, mineval, nevals);
+    flip_ws_free(&F);
+    return a;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Pipeline: T2mapSEcorr! (src/T2mapSEcorr.jl:151-196), voxelwise_T2_distribution! (:201-238),
+ * save_results! (:512-591). CPU worker pool over 64-voxel blocks as split_indices/workerpool
+ * (src/utils.jl:383-468).
+ * ------------------------------------------------------------------------------------------------ */
+typedef struct {
+    const orc_t2map_opts *o;
+    const double *image, *alpha_in;
+    int64_t V; /* nx*ny*nz */
+    int m, n, K, reg;
+    double beta;
+    double *T2, *logT2, *alphas;
+    double *As, *dAs; /* m x n x K grid bases (search), or the fixed basis (K=1) */
+    const int64_t *vox;
+    int64_t nvox;
+    atomic_long next_block;
+    int64_t nblocks;
+    double *dist, *gdn, *ggm, *gva, *fnr, *snr, *alpha, *resnorm, *decaycurve, *mu, *chi2f, *decaybasis;
+} pipe_t;
+
+static void epg_basis(double *A, const pipe_t *P, double alpha) {
+    const orc_t2map_opts *o = P->o;
+    for (int j = 0; j < P->n; ++j) {
+        if (P->beta == 180.0)
+            orc_epg_const(A + (size_t)j * P->m, P->m, alpha, o->TE, P->T2[j], o->T1);
+        else
+            orc_epg_general(A + (size_t)j * P->m, P->m, alpha, o->TE, P->T2[j], o->T1, P->beta);
+    }
+}
+
+static void *pipe_worker(void *arg) {
+    pipe_t *P = arg;
+    const orc_t2map_opts *o = P->o;
+    const int m = P->m, n = P->n;
+    const int64_t V = P->V;
+    reg_ws R;
+    reg_ws_init(&R, m, n);
+    flip_ws F;
+    int search = isnan(o->SetFlipAngle) && P->alpha_in == NULL;
+    flip_ws_init(&F, m, n, P->K);
+    F.As = P->As; F.dAs = P->dAs; F.alphas = P->alphas;
+    double *A = malloc(sizeof(double) * (size_t)m * n), *b = malloc(sizeof(double) * m), *x = malloc(sizeof(double) * n);
+    double *fit = malloc(sizeof(double) * m), *res = malloc(sizeof(double) * m);
+    if (!isnan(o->SetFlipAngle)) memcpy(A, P->As, sizeof(double) * (size_t)m * n);
+    for (;;) {
+        int64_t blk = atomic_fetch_add(&P->next_block, 1);
+        if (blk >= P->nblocks) break;
+        /* split_indices(len, chunks): block blk covers 1 + (blk*len)÷chunks : ((blk+1)*len)÷chunks */
+        int64_t j0 = (blk * P->nvox) / P->nblocks, j1 = ((blk + 1) * P->nvox) / P->nblocks;
+        for (int64_t jj = j0; jj < j1; ++jj) {
+            int64_t I = P->vox[jj];
+            double max_signal = 0.0;
+            for (int i = 0; i < m; ++i) {
+                double bi = P->image[I + V * i];
+                max_signal = bi > max_signal ? bi : max_signal;
+                b[i] = bi;
+            }
+            if (max_signal > 0)
+                for (int i = 0; i < m; ++i) b[i] /= max_signal;
+            double alpha_v;
+            if (P->alpha_in) {
+                alpha_v = P->alpha_in[I];
+                epg_basis(A, P, alpha_v);
+            } else if (search) {
+                alpha_v = flip_search(&F, b, o->nRefAnglesMin, NULL);
+                epg_basis(A, P, alpha_v);
+            } else {
+                alpha_v = o->SetFlipAngle;
+            }
+            reg_bind(&R, A, b);
+            double muv, chi2v;
+            switch (P->reg) {
+            case ORC_REG_NONE: lsq_none(&R, x, &muv, &chi2v); break;
+            case ORC_REG_LCURVE: lsq_lcurve(&R, x, &muv, &chi2v); break;
+            case ORC_REG_GCV: lsq_gcv(&R, x, &muv, &chi2v); break;
+            case ORC_REG_CHI2: lsq_chi2(&R, o->Chi2Factor, x, &muv, &chi2v); break;
+            default: {
+                double sigma = o->NoiseLevel / max_signal;
+                double delta = sqrt((double)m) * sigma;
+                lsq_mdp(&R, delta, x, &muv, &chi2v);
+            }
+            }
+            /* save_results! */
+            for (int i = 0; i < m; ++i) b[i] *= max_signal;
+            for (int j = 0; j < n; ++j) x[j] *= max_signal;
+            for (int i = 0; i < m; ++i) {
+                double s = 0.0;
+                for (int j = 0; j < n; ++j) s += A[i + (size_t)j * m] * x[j];
+                fit[i] = s;
+            }
+            for (int i = 0; i < m; ++i) res[i] = fit[i] - b[i];
+            double sd = 0.0, sr2 = 0.0, mres = 0.0, lg = 0.0;
+            for (int j = 0; j < n; ++j) sd += x[j];
+            for (int i = 0; i < m; ++i) sr2 += res[i] * res[i];
+            for (int i = 0; i < m; ++i) mres += res[i];
+            mres /= m;
+            double var = 0.0;
+            for (int i = 0; i < m; ++i) var += (res[i] - mres) * (res[i] - mres);
+            double sres = sqrt(var / (m - 1));
+            for (int j = 0; j < n; ++j) lg += x[j] * P->logT2[j];
+            double log_ggm = lg / sd;
+            double l1g = 0.0;
+            for (int j = 0; j < n; ++j) l1g += (P->logT2[j] - log_ggm) * (P->logT2[j] - log_ggm) * x[j];
+            l1g /= sd;
+            if (P->gdn) P->gdn[I] = sd;
+            if (P->ggm) P->ggm[I] = exp(log_ggm);
+            if (P->gva) P->gva[I] = expm1(l1g);
+            if (P->fnr) P->fnr[I] = sd / sqrt(sr2 / (m - 1));
+            if (P->snr) P->snr[I] = max_signal / sres;
+            if (P->alpha) P->alpha[I] = alpha_v;
+            if (P->dist)
+                for (int j = 0; j < n; ++j) P->dist[I + V * j] = x[j];
+            if (P->mu) P->mu[I] = muv;
+            if (P->chi2f) P->chi2f[I] = chi2v;
+            if (P->resnorm) P->resnorm[I] = sqrt(sr2);
+            if (P->decaycurve)
+                for (int i = 0; i < m; ++i) P->decaycurve[I + V * i] = fit[i];
+            if (P->decaybasis && isnan(o->SetFlipAngle))
+                for (int j = 0; j < n; ++j)
+                    for (int i = 0; i < m; ++i) P->decaybasis[I + V * (i + (int64_t)m * j)] = A[i + (size_t)j * m];
+        }
+    }
+    free(A); free(b); free(x); free(fit); free(res);
+    flip_ws_free(&F);
+    reg_ws_free(&R);
+    return NULL;
+}
+
+static int pipe_setup(pipe_t *P, const orc_t2map_opts *o, const double *image, const double *alpha_in) {
+    memset(P, 0, sizeof *P);
+    if (o->nTE < 4 || o->nT2 < 2 || !(o->TE > 0) || !(0 < o->T2Range[0] && o->T2Range[0] < o->T2Range[1])) return -1;
+    P->o = o; P->image = image; P->alpha_in = alpha_in;
+    P->V = o->nx * o->ny * o->nz;
+    P->m = o->nTE; P->n = o->nT2; P->reg = o->Reg; P->beta = o->RefConAngle;
+    P->K = isnan(o->SetFlipAngle) ? o->nRefAngles : 1;
+    P->T2 = malloc(sizeof(double) * P->n);
+    P->logT2 = malloc(sizeof(double) * P->n);
+    P->alphas = malloc(sizeof(double) * P->K);
+    orc_logrange(o->T2Range[0], o->T2Range[1], P->n, P->T2);
+    for (int j = 0; j < P->n; ++j) P->logT2[j] = log(P->T2[j]);
+    if (isnan(o->SetFlipAngle)) orc_linrange(o->MinRefAngle, 180.0, P->K, P->alphas);
+    else P->alphas[0] = o->SetFlipAngle;
+    size_t mn = (size_t)P->m * P->n;
+    P->As = malloc(sizeof(double) * mn * P->K);
+    P->dAs = malloc(sizeof(double) * mn * P->K);
+    if (isnan(o->SetFlipAngle)) {
+        for (int k = 0; k < P->K; ++k)
+            for (int j = 0; j < P->n; ++j)
+                orc_epg_dalpha(P->As + mn * k + (size_t)j * P->m, P->dAs + mn * k + (size_t)j * P->m, P->m,
+                               P->alphas[k], o->TE, P->T2[j], o->T1, P->beta);
+    } else {
+        epg_basis(P->As, P, o->SetFlipAngle);
+    }
+    return 0;
+}
+static void pipe_free(pipe_t *P) { free(P->T2); free(P->logT2); free(P->alphas); free(P->As); free(P->dAs); }
+
+static void pipe_run(pipe_t *P, int nthreads) {
+    /* split_indices(len; minchunksize = 64) (src/utils.jl:446-456) */
+    int64_t len = P->nvox;
+    P->nblocks = len >= 64 ? len / 64 : 1;
+    atomic_store(&P->next_block, 0);
+    if (nthreads <= 1) { pipe_worker(P); return; }
+    pthread_t *th = malloc(sizeof(pthread_t) * nthreads);
+    for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, pipe_worker, P);
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    free(th);
+}
+
+int64_t orc_t2map(const orc_t2map_opts *o, const double *image, const double *alpha_in, double *dist, double *gdn,
+                  double *ggm, double *gva, double *fnr, double *snr, double *alpha, double *resnorm,
+                  double *decaycurve, double *mu, double *chi2factor, double *decaybasis, double *decaybasisset,
+                  int nthreads) {
+    pipe_t P;
+    if (pipe_setup(&P, o, image, alpha_in) != 0) return -1;
+    if (decaybasisset) memcpy(decaybasisset, P.As, sizeof(double) * (size_t)P.m * P.n * P.K);
+    if (decaybasis && !isnan(o->SetFlipAngle)) memcpy(decaybasis, P.As, sizeof(double) * (size_t)P.m * P.n);
+    int64_t *vox = malloc(sizeof(int64_t) * (P.V > 0 ? P.V : 1));
+    int64_t nv = 0;
+    for (int64_t I = 0; I < P.V; ++I)
+        if (image[I] > o->Threshold) vox[nv++] = I;
+    P.vox = vox; P.nvox = nv;
+    P.dist = dist; P.gdn = gdn; P.ggm = ggm; P.gva = gva; P.fnr = fnr; P.snr = snr; P.alpha = alpha;
+    P.resnorm = resnorm; P.decaycurve = decaycurve; P.mu = mu; P.chi2f = chi2factor; P.decaybasis = decaybasis;
+    if (nv > 0) pipe_run(&P, nthreads);
+    free(vox);
+    pipe_free(&P);
+    return nv;
+}
+
+int64_t orc_t2map_subset(const orc_t2map_opts *o, const double *image, const int64_t *vox, int64_t nvox, double *dist,
+                         double *gdn, double *ggm, double *gva, double *fnr, double *snr, double *alpha, double *mu,
+                         double *chi2factor, int nthreads) {
+    pipe_t P;
+    if (pipe_setup(&P, o, image, NULL) != 0) return -1;
+    P.vox = vox; P.nvox = nvox;
+    P.dist = dist; P.gdn = gdn; P.ggm = ggm; P.gva = gva; P.fnr = fnr; P.snr = snr; P.alpha = alpha;
+    P.mu = mu; P.chi2f = chi2factor;
+    if (nvox > 0) pipe_run(&P, nthreads);
+    pipe_free(&P);
+    return nvox;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * T2partSEcorr (src/T2partSEcorr.jl:40-164)
+ * ------------------------------------------------------------------------------------------------ */
+int orc_t2part(const orc_t2part_opts *o, const double *dist, double *sfr, double *sgm, double *mfr, double *mgm) {
+    int n = o->nT2;
+    int64_t V = o->nx * o->ny * o->nz;
+    double *T2 = malloc(sizeof(double) * n), *lg = malloc(sizeof(double) * n), *wts = NULL;
+    orc_logrange(o->T2Range[0], o->T2Range[1], n, T2);
+    for (int j = 0; j < n; ++j) lg[j] = log(T2[j]);
+    /* sp_range = findfirst(>=(lo)):findlast(<=(hi)) (1-based); an absent findfirst is an error upstream */
+    int sp0 = -1, sp1 = -1, mp0 = -1, mp1 = -1;
+    for (int j = 0; j < n; ++j) if (sp0 < 0 && T2[j] >= o->SPWin[0]) sp0 = j;
+    for (int j = n - 1; j >= 0; --j) if (sp1 < 0 && T2[j] <= o->SPWin[1]) sp1 = j;
+    for (int j = 0; j < n; ++j) if (mp0 < 0 && T2[j] >= o->MPWin[0]) mp0 = j;
+    for (int j = n - 1; j >= 0; --j) if (mp1 < 0 && T2[j] <= o->MPWin[1]) mp1 = j;
+    if (sp0 < 0 || sp1 < 0 || mp0 < 0 || mp1 < 0) { free(T2); free(lg); return -1; }
+    if (!isnan(o->Sigmoid)) {
+        /* sigmoid_weights (:155-164) */
+        wts = malloc(sizeof(double) * n);
+        double k = 0.1, sigma = fabs(o->Sigmoid / (sqrt(2.0) * orc_erfinv(2 * k - 1)));
+        for (int j = 0; j < n; ++j) {
+            double z = (T2[j] - o->SPWin[1]) / sigma;
+            double v = erfc(z / sqrt(2.0)) / 2;
+            wts[j] = v <= EPS64 ? 0.0 : v;
+        }
+    }
+    for (int64_t I = 0; I < V; ++I) {
+        int bad = 0;
+        for (int j = 0; j < n; ++j) if (isnan(dist[I + V * j])) { bad = 1; break; }
+        if (bad) continue;
+        double S = 0, Ssp = 0, Smp = 0, dsp = 0, dmp = 0;
+        for (int j = 0; j < n; ++j) S += dist[I + V * j];
+        for (int j = sp0; j <= sp1; ++j) { dsp += dist[I + V * j] * lg[j]; Ssp += dist[I + V * j]; }
+        for (int j = mp0; j <= mp1; ++j) { dmp += dist[I + V * j] * lg[j]; Smp += dist[I + V * j]; }
+        if (S > 0) {
+            if (wts) {
+                double d = 0;
+                for (int j = 0; j < n; ++j) d += dist[I + V * j] * wts[j];
+                sfr[I] = d / S;
+            } else {
+                sfr[I] = Ssp / S;
+            }
+            mfr[I] = Smp / S;
+        }
+        if (Ssp > 0) sgm[I] = exp(dsp / Ssp);
+        if (Smp > 0) mgm[I] = exp(dmp / Smp);
+    }
+    free(T2); free(lg); free(wts);
+    return 0;
+}

@@ -1,0 +1,135 @@
+/*
+ * decaes_oracle.h -- FP64 CPU restatement of DECAES.jl's voxelwise T2mapSEcorr / T2partSEcorr path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the parity checker and the timed CPU baseline. It is never
+ * linked into, loaded by, or called from the product library (decaes.jl_amd/). Only tests/, the smoke()
+ * entry and bench.py's CPU-baseline leg may load it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to the DECAES.jl v0.6.1 tree).
+ * Julia is 1-based and column-major; this restatement keeps column-major storage and converts indices.
+ * It is compiled with -ffp-contract=off and calls fma() exactly where the reference uses muladd/@muladd.
+ */
+#ifndef DECAES_ORACLE_H
+#define DECAES_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Regularization methods (src/T2mapSEcorr.jl:440-449). */
+enum { ORC_REG_NONE = 0, ORC_REG_LCURVE = 1, ORC_REG_GCV = 2, ORC_REG_CHI2 = 3, ORC_REG_MDP = 4 };
+
+/* Mirrors T2mapOptions (src/types.jl:19-100). NaN in SetFlipAngle means "nothing". */
+typedef struct {
+    int64_t nx, ny, nz;
+    int32_t nTE;
+    int32_t nT2;
+    double TE;
+    double T2Range[2];
+    double T1;
+    double Threshold;
+    double MinRefAngle;
+    int32_t nRefAngles;
+    int32_t nRefAnglesMin;
+    int32_t Reg;
+    int32_t _pad0;
+    double Chi2Factor;
+    double NoiseLevel;
+    double RefConAngle;
+    double SetFlipAngle;
+} orc_t2map_opts;
+
+/* Mirrors T2partOptions (src/types.jl:139-172). NaN in Sigmoid means "nothing". */
+typedef struct {
+    int64_t nx, ny, nz;
+    int32_t nT2;
+    int32_t _pad0;
+    double T2Range[2];
+    double SPWin[2];
+    double MPWin[2];
+    double Sigmoid;
+} orc_t2part_opts;
+
+/* ---- grids (src/utils.jl:7, src/types.jl:108-109) ---- */
+void orc_logrange(double a, double b, int n, double *out);
+void orc_linrange(double a, double b, int n, double *out);
+double orc_sind(double x);
+double orc_cosd(double x);
+double orc_erfinv(double x);
+
+/* ---- EPG (src/EPGdecaycurve.jl) ---- */
+/* Constant-flip-angle train (EPGWork_ReIm_DualFlat_Split_Dynamic, :936-1028). */
+void orc_epg_const(double *dc, int ETL, double alpha, double TE, double T2, double T1);
+/* General refocusing control angle beta (EPGWork_ReIm_DualVector_Split_Dynamic, :722-818). */
+void orc_epg_general(double *dc, int ETL, double alpha, double TE, double T2, double T1, double beta);
+/* Readable complex rotation-matrix form (EPGWork_Basic_Cplx, :271-322), the reference's own test reference. */
+void orc_epg_basic(double *dc, int ETL, double alpha, double TE, double T2, double T1, double beta);
+/* Value and d/d(alpha) (alpha in degrees), forward mode as ForwardDiff does (:224-248).
+ * beta >= 180 selects the constant-flip recursion, otherwise the general one (src/T2mapSEcorr.jl:616-620). */
+void orc_epg_dalpha(double *dc, double *ddc, int ETL, double alpha, double TE, double T2, double T1, double beta);
+
+/* ---- NNLS (src/NNLS.jl, src/lsqnonneg.jl:30-164) ---- */
+/* Unregularized NNLSProblem solve with the reference warm start. A is m x n column-major.
+ * Outputs x (n), returns rnorm; nsetp, mode and idx (1-based, n) optional (may be NULL). */
+double orc_nnls(const double *A, const double *b, int m, int n, double *x, int *nsetp, int *mode, int *idx,
+                double *w_out);
+/* Tikhonov NNLSProblem solve: min ||Ax-b||^2 + mu^2||x||^2. Returns rnorm of the augmented system. */
+double orc_nnls_tikh(const double *A, const double *b, int m, int n, double mu, double *x, int *nsetp, int *mode);
+
+/* ---- regularization (src/lsqnonneg.jl) ---- */
+/* Each returns x (n) and writes mu, chi2factor. */
+void orc_lsqnonneg_none(const double *A, const double *b, int m, int n, double *x, double *mu, double *chi2);
+void orc_lsqnonneg_chi2(const double *A, const double *b, int m, int n, double chi2_target, double *x, double *mu,
+                        double *chi2);
+void orc_lsqnonneg_mdp(const double *A, const double *b, int m, int n, double delta, double *x, double *mu,
+                       double *chi2);
+void orc_lsqnonneg_gcv(const double *A, const double *b, int m, int n, double *x, double *mu, double *chi2);
+void orc_lsqnonneg_lcurve(const double *A, const double *b, int m, int n, double *x, double *mu, double *chi2);
+/* Singular values, descending, min(m,n) of them (one-sided Jacobi; stands in for LAPACK dgesdd). */
+void orc_svdvals(const double *A, int m, int n, double *s);
+double orc_gcv_dof(int m, int n, const double *gamma, int ngamma, double lambda);
+
+/* ---- scalar optimizers (src/optimization.jl) with a C callback ---- */
+typedef double (*orc_fun1)(double x, void *ctx);
+void orc_brent_root(orc_fun1 f, void *ctx, double x0, double x1, double xatol, double xrtol, double ftol,
+                    int maxiters, double *xout, double *fout);
+void orc_bracket_root_monotonic(orc_fun1 f, void *ctx, double a, double delta, double dilate, double mono,
+                                int maxiters, double out[4]);
+void orc_brent_minimize(orc_fun1 f, void *ctx, double x1, double x2, double xrtol, double xatol, int maxiters,
+                        double *xout, double *fout);
+/* lcurve_corner on a caller-supplied curve f(logmu) -> (xi, eta) (src/lsqnonneg.jl:872-972). */
+typedef void (*orc_fun2)(double x, void *ctx, double out[2]);
+double orc_lcurve_corner(orc_fun2 f, void *ctx, double xlow, double xhigh, double xtol, double Ptol, double Ctol);
+/* Largest L-curve cache sizes (function, point, state caches) seen since the previous call. */
+void orc_lcurve_cache_stats(int out[3]);
+/* CubicHermiteInterpolator + minimize (src/splines.jl:62-110). out = (x, u). */
+void orc_hermite_minimize(double a, double b, double u0, double u1, double m0, double m1, double out[2]);
+
+/* ---- flip-angle search on one voxel (src/T2mapSEcorr.jl:409-423, src/splines.jl:664-850,975-1051) ---- */
+/* As: m x n x K grid bases, dAs: their d/dalpha. Returns alpha_opt; nevals optional. */
+double orc_flip_search(const double *As, const double *dAs, const double *alphas, int m, int n, int K, int mineval,
+                       const double *b, int *nevals);
+
+/* ---- whole pipeline (src/T2mapSEcorr.jl:148-238,512-591) ---- */
+/* image: (nx,ny,nz,nTE) column-major. alpha_in: B1 map (nx,ny,nz) or NULL. Outputs are caller-allocated and
+ * must be pre-filled with NaN; voxels below threshold are left untouched. Optional outputs may be NULL.
+ * decaybasisset (nTE x nT2 x K, or nTE x nT2 with SetFlipAngle) may be NULL.
+ * Returns the number of processed voxels, or a negative error code. */
+int64_t orc_t2map(const orc_t2map_opts *o, const double *image, const double *alpha_in, double *dist, double *gdn,
+                  double *ggm, double *gva, double *fnr, double *snr, double *alpha, double *resnorm,
+                  double *decaycurve, double *mu, double *chi2factor, double *decaybasis, double *decaybasisset,
+                  int nthreads);
+/* Pipeline over an explicit list of linear voxel indices (used for bounded CPU-baseline samples). */
+int64_t orc_t2map_subset(const orc_t2map_opts *o, const double *image, const int64_t *vox, int64_t nvox,
+                         double *dist, double *gdn, double *ggm, double *gva, double *fnr, double *snr,
+                         double *alpha, double *mu, double *chi2factor, int nthreads);
+
+/* src/T2partSEcorr.jl:40-164. dist: (nx,ny,nz,nT2). Outputs pre-filled with NaN. */
+int orc_t2part(const orc_t2part_opts *o, const double *dist, double *sfr, double *sgm, double *mfr, double *mgm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,174 @@
+"""GPU parity: the gfx950 path through the C ABI against the FP64 oracle on identical seeded inputs.
+
+Tolerances are the plain FP64 path's: alpha <= 0.01 deg, distribution relative L2 <= 1e-6, MWF (sfr) absolute
+<= 1e-6 per voxel. Both sides reduce in different orders (the reference's @simd reductions reassociate too), so
+a voxel whose flip-angle bisection or regularization search sits on a branch tie may take the other branch; the
+tests bound the fraction of such voxels and print the full error distribution.
+"""
+import json
+import math
+import os
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+OUT = Path(os.environ.get("GRAFT_REPO_ROOT", Path(__file__).resolve().parents[1])) / "gpurun_out"
+
+CASES = {
+    # name: (shape, nTE, TE, nT2, T2Range, Reg, nRefAngles, extra options, ncomp, mask fraction)
+    "c1_chi2": ((16, 16, 2), 32, 10e-3, 40, (10e-3, 2.0), "chi2", 32, dict(Chi2Factor=1.02), 3, 1.0),
+    "c2_gcv": ((12, 12, 3), 48, 10e-3, 40, (10e-3, 2.0), "gcv", 32, {}, 3, 0.6),
+    "c3_lcurve": ((10, 10, 2), 56, 7.9e-3, 60, (8e-3, 2.0), "lcurve", 64, {}, 4, 0.6),
+    "none": ((10, 10, 2), 32, 10e-3, 40, (10e-3, 2.0), "none", 32, {}, 3, 1.0),
+    "mdp": ((10, 10, 2), 48, 10e-3, 40, (10e-3, 2.0), "mdp", 32, dict(NoiseLevel=2e-3), 3, 1.0),
+    "m24_gcv_n40": ((8, 8, 2), 24, 7e-3, 40, (10e-3, 2.0), "gcv", 64, {}, 3, 1.0),
+}
+
+
+def _run(dmi, oracle, name, seed=11, **over):
+    shape, m, TE, n, T2R, reg, K, extra, ncomp, frac = CASES[name]
+    img, nin = dmi.synth_mse(shape, m, TE, ncomp=ncomp, mask_fraction=frac, seed=seed)
+    kw = dict(TE=TE, nT2=n, T2Range=T2R, Reg=reg, nRefAngles=K, SaveRegParam=True, SaveResidualNorm=True,
+              SaveDecayCurve=True, Silent=True, **extra)
+    kw.update(over)
+    opts = dmi.T2mapOptions.from_image(img, **kw)
+    maps, dist = dmi.T2mapSEcorr(img, opts)
+    oo = oracle.make_t2map_opts(shape, m, TE, n, T2R, reg, nRefAngles=K, nRefAnglesMin=opts.nRefAnglesMin,
+                                Chi2Factor=extra.get("Chi2Factor"), NoiseLevel=extra.get("NoiseLevel"),
+                                RefConAngle=over.get("RefConAngle", 180.0), SetFlipAngle=over.get("SetFlipAngle"))
+    omaps, odist, nv = oracle.t2map(img, oo, nthreads=os.cpu_count())
+    return img, opts, maps, dist, omaps, odist, nv
+
+
+def _errors(maps, dist, omaps, odist, T2R):
+    mask = ~np.isnan(omaps["gdn"])
+    d_g = dist[mask]
+    d_o = odist[mask]
+    rel = np.linalg.norm(d_g - d_o, axis=1) / np.maximum(np.linalg.norm(d_o, axis=1), 1e-300)
+    da = np.abs(maps["alpha"][mask] - omaps["alpha"][mask])
+    return mask, rel, da
+
+
+def _summary(name, mask, rel, da, mwf_err, maps, omaps):
+    def q(x):
+        x = np.asarray(x)
+        return dict(max=float(np.max(x)), p99=float(np.quantile(x, 0.99)), median=float(np.median(x)))
+
+    s = dict(case=name, voxels=int(mask.sum()), dist_rel_l2=q(rel), alpha_abs=q(da), mwf_abs=q(mwf_err),
+             mu_rel=q(np.abs(maps["mu"][mask] - omaps["mu"][mask]) / np.maximum(np.abs(omaps["mu"][mask]), 1e-300)),
+             frac_ok=float(np.mean((rel <= 1e-6) & (da <= 1e-2) & (mwf_err <= 1e-6))))
+    OUT.mkdir(exist_ok=True)
+    with open(OUT / f"parity_{name}.json", "w") as f:
+        json.dump(s, f, indent=1)
+    print(json.dumps(s), flush=True)
+    return s
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_t2map_matches_oracle(dmi, oracle, name):
+    print(f"[parity] case={name}", flush=True)
+    img, opts, maps, dist, omaps, odist, nv = _run(dmi, oracle, name)
+    shape, m, TE, n, T2R = CASES[name][:5]
+    # identical voxel selection and NaN placement
+    assert np.array_equal(np.isnan(maps["gdn"]), np.isnan(omaps["gdn"]))
+    assert np.array_equal(np.isnan(dist), np.isnan(odist))
+    mask, rel, da = _errors(maps, dist, omaps, odist, T2R)
+    pg = oracle.t2part(dist, T2R, (T2R[0], 40e-3), (40e-3, T2R[1]))
+    po = oracle.t2part(odist, T2R, (T2R[0], 40e-3), (40e-3, T2R[1]))
+    mwf_err = np.abs(pg["sfr"][mask] - po["sfr"][mask])
+    s = _summary(name, mask, rel, da, mwf_err, maps, omaps)
+    # the flip-angle search runs before regularization: alpha always agrees to the FP64 bar
+    assert np.max(da) <= 1e-2, s
+    if opts.Reg == "lcurve":
+        # The L-curve corner search is discontinuous at the rounding level: its isapprox-keyed caches, backtracking
+        # and curvature comparisons pick a corner up to ~4 xtol away after a 1-ulp change of the input. The bar is
+        # therefore the oracle's own sensitivity: perturb the input by 1 ulp, rerun the oracle, and require the GPU
+        # to differ from the oracle no more often and no further than the oracle differs from itself.
+        img2 = img * (1 + 2.0 ** -52)
+        oo = oracle.make_t2map_opts(shape, m, TE, n, T2R, "lcurve", nRefAngles=opts.nRefAngles)
+        pmaps, pdist, _ = oracle.t2map(img2, oo, nthreads=os.cpu_count())
+        self_rel = np.linalg.norm(pdist[mask] - odist[mask], axis=1) / np.linalg.norm(odist[mask], axis=1)
+        self_dlog = np.abs(np.log(pmaps["mu"][mask]) - np.log(omaps["mu"][mask]))
+        dlog = np.abs(np.log(maps["mu"][mask]) - np.log(omaps["mu"][mask]))
+        print(f"[parity] lcurve self-sensitivity: frac dist>1e-6 {np.mean(self_rel > 1e-6):.4f}, "
+              f"max dlogmu {self_dlog.max():.2e}; gpu: frac {np.mean(rel > 1e-6):.4f}, max dlogmu {dlog.max():.2e}")
+        assert np.mean(rel > 1e-6) <= np.mean(self_rel > 1e-6) + 0.05, s
+        assert dlog.max() <= max(2 * self_dlog.max(), 1e-3), s
+        assert np.max(mwf_err) <= 1e-3, s
+    else:
+        assert s["frac_ok"] >= 0.97, s
+    # per-voxel maps that depend only on the solution agree to the same tolerance
+    for k in ("gdn", "fnr", "snr", "resnorm", "chi2factor"):
+        a, b = maps[k][mask], omaps[k][mask]
+        assert np.mean(np.abs(a - b) <= 1e-6 * np.maximum(np.abs(b), 1.0)) >= 0.97, k
+
+
+def test_decaybasisset_matches_oracle(dmi, oracle):
+    opts = dmi.T2mapOptions(MatrixSize=(1, 1, 1), nTE=48, TE=10e-3, nT2=40, T2Range=(10e-3, 2.0), Reg="gcv",
+                            nRefAngles=32, Silent=True)
+    g = dmi.decaybasisset(opts)
+    oo = oracle.make_t2map_opts((1, 1, 1), 48, 10e-3, 40, (10e-3, 2.0), "gcv", nRefAngles=32)
+    img = np.zeros((1, 1, 1, 48), order="F")
+    omaps, _, _ = oracle.t2map(img, oo, nthreads=1)
+    assert g.shape == (48, 40, 32)
+    assert np.max(np.abs(g - omaps["decaybasisset"])) <= 1e-13
+
+
+def test_general_refcon_angle(dmi, oracle):
+    print("[parity] case=refcon172", flush=True)
+    img, opts, maps, dist, omaps, odist, nv = _run(dmi, oracle, "none", RefConAngle=172.0)
+    mask, rel, da = _errors(maps, dist, omaps, odist, (10e-3, 2.0))
+    assert np.mean((rel <= 1e-6) & (da <= 1e-2)) >= 0.97
+
+
+def test_set_flip_angle(dmi, oracle):
+    print("[parity] case=setflip", flush=True)
+    img, opts, maps, dist, omaps, odist, nv = _run(dmi, oracle, "c1_chi2", SetFlipAngle=165.0)
+    mask, rel, da = _errors(maps, dist, omaps, odist, (10e-3, 2.0))
+    assert np.all(maps["alpha"][mask] == 165.0)
+    assert np.mean(rel <= 1e-6) >= 0.97
+
+
+def test_b1_map(dmi, oracle):
+    print("[parity] case=b1map", flush=True)
+    shape, m, TE, n, T2R = (8, 8, 2), 32, 10e-3, 40, (10e-3, 2.0)
+    img, _ = dmi.synth_mse(shape, m, TE, mask_fraction=0.7, seed=5)
+    b1 = np.asfortranarray(np.random.default_rng(0).uniform(120, 180, shape))
+    opts = dmi.T2mapOptions.from_image(img, TE=TE, nT2=n, T2Range=T2R, Reg="none", nRefAngles=32, Silent=True)
+    maps, dist = dmi.T2mapSEcorr(img, opts, alpha=b1)
+    oo = oracle.make_t2map_opts(shape, m, TE, n, T2R, "none", nRefAngles=32)
+    omaps, odist, _ = oracle.t2map(img, oo, alpha_in=b1)
+    # alpha echoes the B1 map everywhere (also below threshold, as load_B1map! wrote it)
+    assert np.array_equal(maps["alpha"], b1)
+    mask, rel, da = _errors(maps, dist, omaps, odist, T2R)
+    assert np.mean(rel <= 1e-6) >= 0.97
+
+
+def test_t2part_matches_oracle(dmi, oracle):
+    rng = np.random.default_rng(3)
+    dist = np.asfortranarray(rng.uniform(0, 1, (9, 7, 3, 40)) * (rng.uniform(0, 1, (9, 7, 3, 40)) > 0.5))
+    dist[0, 0, 0, 5] = np.nan
+    dist[1, 0, 0, :] = 0.0
+    for sig in (None, 5e-3):
+        g = dmi.T2partSEcorr(dist, T2Range=(10e-3, 2.0), SPWin=(10e-3, 40e-3), MPWin=(40e-3, 2.0), Sigmoid=sig,
+                             Silent=True)
+        o = oracle.t2part(dist, (10e-3, 2.0), (10e-3, 40e-3), (40e-3, 2.0), Sigmoid=sig)
+        for k in ("sfr", "sgm", "mfr", "mgm"):
+            assert np.array_equal(np.isnan(g[k]), np.isnan(o[k])), k
+            ok = ~np.isnan(o[k])
+            assert np.max(np.abs(g[k][ok] - o[k][ok]) / np.maximum(np.abs(o[k][ok]), 1e-300)) <= 1e-12, k
+
+
+def test_threshold_edge_cases(dmi):
+    shape = (4, 4, 1)
+    img = np.zeros((*shape, 32), order="F")
+    opts = dmi.T2mapOptions.from_image(img, TE=10e-3, nT2=40, T2Range=(10e-3, 2.0), Reg="none", Silent=True)
+    maps, dist = dmi.T2mapSEcorr(img, opts)
+    assert np.all(np.isnan(maps["gdn"])) and np.all(np.isnan(dist))
+    img2, _ = dmi.synth_mse(shape, 32, 10e-3, mask_fraction=1.0, seed=1)
+    img2[0, 0, 0, 3] = np.nan  # a NaN echo propagates to that voxel's outputs, others unaffected
+    maps2, dist2 = dmi.T2mapSEcorr(img2, opts)
+    assert np.sum(~np.isnan(maps2["gdn"])) >= shape[0] * shape[1] - 1
