@@ -1,0 +1,161 @@
+"""Throughput of the gfx950 T2mapSEcorr + T2partSEcorr path on one GPU.
+
+One step = T2mapSEcorr over the whole 240x240x48 48-echo volume (flip-angle search, EPG bases, GCV-regularized
+NNLS, maps) followed by T2partSEcorr on its distribution, with the image and all outputs resident in HBM.
+value = processed (above-threshold) voxels per second. Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "decaes.jl_amd"))
+
+WORKLOADS = {
+    # plain FP64 drop-in volume: 240x240x48, nTE 48, TE 10 ms, nT2 40 over 10 ms-2 s, gcv, 32 angles, 60% mask
+    "c2": dict(shape=(240, 240, 48), nTE=48, TE=10e-3, nT2=40, T2Range=(10e-3, 2.0), Reg="gcv", nRefAngles=32,
+               ncomp=3, mask=0.6, Chi2Factor=None),
+    # reference-CPU-sized case: 32x32x4, nTE 32, chi2 1.02, 32 angles, every voxel
+    "c1": dict(shape=(32, 32, 4), nTE=32, TE=10e-3, nT2=40, T2Range=(10e-3, 2.0), Reg="chi2", nRefAngles=32,
+               ncomp=3, mask=1.0, Chi2Factor=1.02),
+    # 240x240x113, nTE 56, TE 7.9 ms, nT2 60 over 8 ms-2 s, lcurve, 64 angles, 4 compartments
+    "c3": dict(shape=(240, 240, 113), nTE=56, TE=7.9e-3, nT2=60, T2Range=(8e-3, 2.0), Reg="lcurve", nRefAngles=64,
+               ncomp=4, mask=0.6, Chi2Factor=None),
+}
+SPWIN = (None, 40e-3)  # short-peak window upper edge (lower edge = T2Range[0])
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(img, opts_kw, seconds: float):
+    """The FP64 oracle on host cores over a bounded, seeded sample of the same in-mask voxels."""
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib
+
+    oracle_lib.build()
+    w = opts_kw
+    shape = img.shape[:3]
+    oo = oracle_lib.make_t2map_opts(shape, w["nTE"], w["TE"], w["nT2"], w["T2Range"], w["Reg"],
+                                    nRefAngles=w["nRefAngles"], Chi2Factor=w["Chi2Factor"])
+    first = img[..., 0].reshape(-1, order="F")
+    inmask = np.flatnonzero(first > 0.0)
+    rng = np.random.default_rng(1234)
+    order = rng.permutation(inmask)
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(threads, 16))
+    done, t0, chunk = 0, time.perf_counter(), 256 * threads
+    while time.perf_counter() - t0 < seconds and done < len(order):
+        vox = np.sort(order[done:done + chunk])
+        oracle_lib.t2map_subset(img, oo, vox, nthreads=threads)
+        done += len(vox)
+    dt = time.perf_counter() - t0
+    return dict(value=done / dt, unit="voxels/s", cores=threads, kind="port",
+                sample=f"{done} in-mask voxels drawn (seed 1234) from the same volume, {dt:.1f} s of CPU work")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); voxels shard across GPUs with no exchange "
+              "step, so a multi-GPU figure is N independent single-GPU runs.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import decaes_amd as dmi
+    from decaes_amd.device import alloc_t2map_outputs, t2map_device, t2part_device
+
+    w = WORKLOADS[args.workload]
+    X, Y, Z = w["shape"]
+    log(f"[bench] synthesizing {args.workload} {w['shape']} nTE={w['nTE']}")
+    t = time.perf_counter()
+    img, nin = dmi.synth_mse(w["shape"], w["nTE"], w["TE"], ncomp=w["ncomp"], mask_fraction=w["mask"], seed=2024)
+    log(f"[bench] synth done in {time.perf_counter() - t:.1f}s, {nin} voxels in mask")
+    opts = dmi.T2mapOptions(MatrixSize=w["shape"], nTE=w["nTE"], TE=w["TE"], nT2=w["nT2"], T2Range=w["T2Range"],
+                            Reg=w["Reg"], nRefAngles=w["nRefAngles"], Chi2Factor=w["Chi2Factor"], Silent=True)
+    popts = dmi.T2partOptions.from_t2map(opts, SPWin=(w["T2Range"][0], 40e-3), MPWin=(40e-3, w["T2Range"][1]))
+    dev = torch.device("cuda:0")
+    # (X,Y,Z,nTE) column-major == contiguous (nTE,Z,Y,X)
+    d_img = torch.from_numpy(np.ascontiguousarray(img.transpose(3, 2, 1, 0))).to(dev)
+    out = alloc_t2map_outputs(opts, dev)
+    pout = {k: torch.full((Z, Y, X), float("nan"), dtype=torch.float64, device=dev) for k in ("sfr", "sgm", "mfr", "mgm")}
+    ctx = dmi.context(0)
+    torch.cuda.synchronize()
+
+    def step():
+        st = t2map_device(d_img, opts, out, ctx=ctx)
+        t2part_device(out["dist"], popts, pout, ctx=ctx)
+        return st
+
+    for i in range(args.warmup):
+        st = step()
+        log(f"[bench] warmup {i}: kernel {st['kernel_ms']:.1f} ms, {st['voxels']} voxels")
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = step()
+        stats.append(st)
+        log(f"[bench] step {i}: kernel {st['kernel_ms']:.1f} ms, total {st['total_ms']:.1f} ms")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    nvox = stats[0]["voxels"]
+    ms_step = 1e3 * dt / args.steps
+    kernel_ms = float(np.mean([s["kernel_ms"] for s in stats]))
+    flops = float(np.mean([s["fp64_flops"] for s in stats]))
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    # sanity: outputs are finite in the mask
+    gdn = out["gdn"].cpu().numpy()
+    finite = int(np.isfinite(gdn).sum())
+    res = {
+        "metric": "voxels/s (T2map+T2part)",
+        "value": nvox / (dt / args.steps),
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {X}x{Y}x{Z} nTE={w['nTE']} TE={w['TE']} nT2={w['nT2']} "
+                               f"T2Range={w['T2Range']} Reg={w['Reg']} nRefAngles={w['nRefAngles']} "
+                               f"mask={w['mask']} precision=fp64",
+                   "voxels_processed": nvox, "voxels_volume": X * Y * Z, "finite_gdn": finite,
+                   "nnls_solves_per_voxel": stats[0]["nnls_solves"] / max(nvox, 1),
+                   "fp64_flops_per_voxel": flops / max(nvox, 1)},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": f"t2map_kernel<{dmi.api._lib.REG[w['Reg']]}>", "kernel_ms": kernel_ms,
+                     "flops_per_launch": flops},
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline (oracle on host cores)")
+        res["cpu_baseline"] = cpu_baseline(img, dict(w, **{"nTE": w["nTE"]}), args.cpu_seconds)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -65,7 +65,7 @@ struct dmi_ctx {
     dmi_stats stats{};
     // grids
     GridKey key;
-    DevBuf E2, logT2, alphas, As, dAs, Afixed;
+    DevBuf E2, logT2, alphas, As, dAs, Afixed, AsT, dAsT, AfixedT;
     // per-call buffers
     DevBuf vox, counter, scratch, counters, status;
     // host-array staging
@@ -192,14 +192,20 @@ int ensure_grids(dmi_ctx *c, const dmi_t2map_opts *o, bool search) {
     if (search) {
         HIPCHECK(c, c->As.ensure(bytes));
         HIPCHECK(c, c->dAs.ensure(bytes));
+        HIPCHECK(c, c->AsT.ensure(bytes));
+        HIPCHECK(c, c->dAsT.ensure(bytes));
         g.As = c->As.as<double>();
         g.dAs = c->dAs.as<double>();
+        g.AsT = c->AsT.as<double>();
     } else {
         HIPCHECK(c, c->Afixed.ensure(bytes));
+        HIPCHECK(c, c->AfixedT.ensure(bytes));
         g.As = c->Afixed.as<double>();
         g.dAs = nullptr;
+        g.AsT = c->AfixedT.as<double>();
     }
     HIPCHECK(c, dmi::launch_grid_basis(g, c->stream));
+    if (search) HIPCHECK(c, dmi::launch_transpose(c->dAs.as<double>(), c->dAsT.as<double>(), m, n, K, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     c->key = k;
     return DMI_OK;
@@ -266,8 +272,10 @@ int run_t2map(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, const 
     p.logT2 = c->logT2.as<double>();
     p.alphas = c->alphas.as<double>();
     p.As = fixed ? nullptr : c->As.as<double>();
-    p.dAs = fixed ? nullptr : c->dAs.as<double>();
+    p.AsT = fixed ? nullptr : c->AsT.as<double>();
+    p.dAsT = fixed ? nullptr : c->dAsT.as<double>();
     p.Afixed = fixed ? c->Afixed.as<double>() : nullptr;
+    p.AfixedT = fixed ? c->AfixedT.as<double>() : nullptr;
     p.alpha_in = d_alpha_in;
     p.dist = d->dist; p.gdn = d->gdn; p.ggm = d->ggm; p.gva = d->gva; p.fnr = d->fnr; p.snr = d->snr; p.alpha = d->alpha;
     p.resnorm = o->SaveResidualNorm ? d->resnorm : nullptr;
@@ -396,7 +404,8 @@ int dmi_destroy(dmi_ctx *c) {
     if (!c) return DMI_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    DevBuf *bufs[] = {&c->E2, &c->logT2, &c->alphas, &c->As, &c->dAs, &c->Afixed, &c->vox, &c->counter, &c->scratch,
+    DevBuf *bufs[] = {&c->E2, &c->logT2, &c->alphas, &c->As, &c->dAs, &c->Afixed, &c->AsT, &c->dAsT, &c->AfixedT,
+                      &c->vox, &c->counter, &c->scratch,
                       &c->counters, &c->status, &c->img, &c->ain, &c->weights, &c->tlog};
     for (auto *b : bufs) b->release();
     for (auto &b : c->out) b.release();

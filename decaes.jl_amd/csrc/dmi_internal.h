@@ -29,6 +29,7 @@ struct GridParams {
     const double *alphas; // K
     double *As;           // K x m x n, row-major per angle
     double *dAs;          // K x m x n or nullptr
+    double *AsT;          // K x n x m (columns contiguous) or nullptr
 };
 
 struct T2mapParams {
@@ -45,8 +46,10 @@ struct T2mapParams {
     const double *logT2;  // n
     const double *alphas; // K
     const double *As;     // K x m x n row-major grid bases
-    const double *dAs;    // their d/dalpha
+    const double *AsT;    // K x n x m (columns contiguous)
+    const double *dAsT;   // d/dalpha, K x n x m
     const double *Afixed; // m x n row-major (SetFlipAngle)
+    const double *AfixedT;// n x m
     const double *alpha_in;
     double *scratch;      // per-workgroup slots
     int64_t slot_stride;  // doubles per slot
@@ -69,6 +72,7 @@ size_t t2map_lds_bytes(int m, int n, int lda, int mw);
 int64_t t2map_slot_doubles(int m, int n);
 
 hipError_t launch_grid_basis(const GridParams &p, hipStream_t s);
+hipError_t launch_transpose(const double *src, double *dst, int m, int n, int K, hipStream_t s);
 hipError_t launch_select(const double *image, int64_t V, double threshold, int64_t *vox, unsigned int *count,
                          hipStream_t s);
 hipError_t launch_t2map(const T2mapParams &p, int grid, size_t lds, hipStream_t s);

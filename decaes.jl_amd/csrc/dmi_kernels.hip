@@ -24,6 +24,13 @@
 namespace dmi {
 namespace {
 
+// Explicit address spaces: pointers kept in structs otherwise decay to generic (flat) addresses.
+typedef __attribute__((address_space(3))) double ldbl;
+typedef __attribute__((address_space(3))) int lint;
+typedef __attribute__((address_space(1))) double gdbl;
+typedef const __attribute__((address_space(1))) double cgdbl;
+typedef const __attribute__((address_space(1))) int64_t cgi64;
+
 constexpr double kDeg2Rad = 3.141592653589793 / 180.0;  // Julia deg2rad: x * (Float64(pi)/180)
 constexpr double kPi180Hi = 0.017453292519943295;      // pi/180 as a double-double
 constexpr double kPi180Lo = 2.9486522708701687e-19;
@@ -51,41 +58,66 @@ __device__ __forceinline__ double readlane(double v, int l) {
 }
 __device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
+// Reductions: DPP within each 16-lane row (quad swaps, half-row mirror, row mirror), then the four row
+// results combined in a fixed order from readlane. Every lane of a row holds the same partial (each combine
+// is symmetric in its two operands), so the result is deterministic and uniform.
+template <int CTRL> __device__ __forceinline__ int dpp_i(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false); }
+template <int CTRL> __device__ __forceinline__ double dpp_d(double v) {
+    unsigned long long u = __double_as_longlong(v);
+    int lo = dpp_i<CTRL>((int)(u & 0xffffffffull)), hi = dpp_i<CTRL>((int)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+constexpr int kQuadSwap1 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int kQuadSwap2 = 0x4E;  // quad_perm [2,3,0,1]
+constexpr int kHalfMirror = 0x141;
+constexpr int kRowMirror = 0x140;
+template <class Op> __device__ __forceinline__ double row_reduce(double v, Op op) {
+    v = op(v, dpp_d<kQuadSwap1>(v));
+    v = op(v, dpp_d<kQuadSwap2>(v));
+    v = op(v, dpp_d<kHalfMirror>(v));
+    v = op(v, dpp_d<kRowMirror>(v));
+    return v;
+}
 __device__ __forceinline__ double wsum(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return rfl(v);
+    v = row_reduce(v, [](double a, double b) { return a + b; });
+    return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
 }
 __device__ __forceinline__ double wmaxd(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
-    return rfl(v);
+    v = row_reduce(v, [](double a, double b) { return fmax(a, b); });
+    return fmax(fmax(readlane(v, 0), readlane(v, 16)), fmax(readlane(v, 32), readlane(v, 48)));
 }
 __device__ __forceinline__ double wmind(double v) {
+    v = row_reduce(v, [](double a, double b) { return fmin(a, b); });
+    return fmin(fmin(readlane(v, 0), readlane(v, 16)), fmin(readlane(v, 32), readlane(v, 48)));
+}
+// (value, index) pairs; Better(a, ia, b, ib) is a strict total order on pairs.
+template <int CTRL, class Better> __device__ __forceinline__ void pair_step(double &v, int &i, Better better) {
+    double ov = dpp_d<CTRL>(v);
+    int oi = dpp_i<CTRL>(i);
+    if (better(ov, oi, v, i)) { v = ov; i = oi; }
+}
+template <class Better> __device__ __forceinline__ void wpair(double &v, int &i, Better better) {
+    pair_step<kQuadSwap1>(v, i, better);
+    pair_step<kQuadSwap2>(v, i, better);
+    pair_step<kHalfMirror>(v, i, better);
+    pair_step<kRowMirror>(v, i, better);
+    double bv = readlane(v, 0);
+    int bi = readlane(i, 0);
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
-    return rfl(v);
+    for (int r = 16; r < 64; r += 16) {
+        double rv = readlane(v, r);
+        int ri = readlane(i, r);
+        if (better(rv, ri, bv, bi)) { bv = rv; bi = ri; }
+    }
+    v = bv;
+    i = bi;
 }
 // (value desc, index asc) arg-max and (value asc, index asc) arg-min.
 __device__ __forceinline__ void wargmax(double &v, int &i) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        double ov = __shfl_xor(v, o, kWave);
-        int oi = __shfl_xor(i, o, kWave);
-        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
-    }
-    v = rfl(v);
-    i = rfl(i);
+    wpair(v, i, [](double a, int ia, double b, int ib) { return a > b || (a == b && ia < ib); });
 }
 __device__ __forceinline__ void wargmin(double &v, int &i) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        double ov = __shfl_xor(v, o, kWave);
-        int oi = __shfl_xor(i, o, kWave);
-        if (ov < v || (ov == v && oi < i)) { v = ov; i = oi; }
-    }
-    v = rfl(v);
-    i = rfl(i);
+    wpair(v, i, [](double a, int ia, double b, int ib) { return a < b || (a == b && ia < ib); });
 }
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int first_set(unsigned long long m) { return m ? __ffsll((long long)m) - 1 : -1; }
@@ -121,7 +153,7 @@ __device__ __forceinline__ double cos_dd(double x) {
     sincos(hi, &s, &c);
     return fma(-s, lo, c);
 }
-__device__ double dsind(double x) {
+__device__ __forceinline__ double dsind(double x) {
     if (isinf(x) || isnan(x)) return NAN;
     double rx = copysign(fmod(x, 360.0), x), arx = fabs(rx);
     if (rx == 0) return rx;
@@ -132,7 +164,7 @@ __device__ double dsind(double x) {
     if (arx <= 315.0) return -copysign(cos_dd(270.0 - arx), rx);
     return sin_dd(rx - copysign(360.0, rx));
 }
-__device__ double dcosd(double x) {
+__device__ __forceinline__ double dcosd(double x) {
     if (isinf(x) || isnan(x)) return NAN;
     double rx = fabs(fmod(x, 360.0));
     if (rx <= 45.0) return cos_dd(rx);
@@ -170,15 +202,17 @@ template <typename T> struct Num;
 template <> struct Num<double> {
     static __device__ __forceinline__ double c(double v) { return v; }
     static __device__ __forceinline__ double var(double v, double) { return v; }
-    static __device__ __forceinline__ void put(double x, double *v, double *, int64_t o) { v[o] = x; }
+    static __device__ __forceinline__ void put(double x, gdbl *v, gdbl *, int64_t o) { v[o] = x; }
+    static __device__ __forceinline__ double val(double x) { return x; }
 };
 template <> struct Num<Dual> {
     static __device__ __forceinline__ Dual c(double v) { return {v, 0.0}; }
     static __device__ __forceinline__ Dual var(double v, double d) { return {v, d}; }
-    static __device__ __forceinline__ void put(Dual x, double *v, double *d, int64_t o) {
+    static __device__ __forceinline__ void put(Dual x, gdbl *v, gdbl *d, int64_t o) {
         v[o] = x.v;
         if (d) d[o] = x.d;
     }
+    static __device__ __forceinline__ double val(Dual x) { return x.v; }
 };
 template <int LPC> __device__ __forceinline__ double shup(double a) { return __shfl_up(a, 1, LPC); }
 template <int LPC> __device__ __forceinline__ double shdn(double a) { return __shfl_down(a, 1, LPC); }
@@ -192,8 +226,8 @@ template <int LPC> __device__ __forceinline__ Dual shdn(Dual a) { return {__shfl
 //
 // Constant-flip train: EPGConstantFlipAngleOptions, DualFlat recursion (:948-1028) + scaling (:936-946).
 template <int LPC, typename T>
-__device__ void epg_const(double alpha_deg, double E1, double E2, int ETL, int s, bool store, double *out_v,
-                          double *out_d, int64_t ostride) {
+__device__ __forceinline__ void epg_const(double alpha_deg, double E1, double E2, int ETL, int s, bool store, gdbl *out_v,
+                          gdbl *out_d, int64_t ostride, gdbl *outT) {
     using N = Num<T>;
     const T al = N::var(alpha_deg * kDeg2Rad, kDeg2Rad);
     T sn, cs;
@@ -216,7 +250,11 @@ __device__ void epg_const(double alpha_deg, double E1, double E2, int ETL, int s
         } else {
             echo = tfma(mc, Z, tfma(a, C, -(b * S)));
         }
-        if (store && s == 0) N::put(tabs(m0 * echo), out_v, out_d, i * ostride);
+        if (store && s == 0) {
+            const T r = tabs(m0 * echo);
+            N::put(r, out_v, out_d, i * ostride);
+            if (outT) outT[i] = N::val(r);
+        }
     }
 }
 
@@ -226,8 +264,8 @@ template <typename T> __device__ __forceinline__ T dot3(T x0, T x1, T x2, T y0, 
     return (x0 * y0 + x1 * y1) + x2 * y2;
 }
 template <int LPC, typename T>
-__device__ void epg_general(double alpha_deg, double beta, double E1, double E2, int ETL, int s, bool store,
-                            double *out_v, double *out_d, int64_t ostride) {
+__device__ __forceinline__ void epg_general(double alpha_deg, double beta, double E1, double E2, int ETL, int s, bool store,
+                            gdbl *out_v, gdbl *out_d, int64_t ostride, gdbl *outT) {
     using N = Num<T>;
     const T alpha = N::var(alpha_deg, 1.0);
     const T A = alpha / 180.0;
@@ -245,10 +283,18 @@ __device__ void epg_general(double alpha_deg, double beta, double E1, double E2,
     T F = N::c(0.0), Fb = N::c(0.0), Z = N::c(0.0);
     if (s == 0) { F = b1 * m0; Z = ((-c1) * m0) / 2.0; }
     if (s == 1) F = a1 * m0;
-    if (store && s == 0) N::put(tabs(b1 * m0), out_v, out_d, 0);
+    if (store && s == 0) {
+        const T r = tabs(b1 * m0);
+        N::put(r, out_v, out_d, 0);
+        if (outT) outT[0] = N::val(r);
+    }
     for (int i = 1; i < ETL; ++i) {
         const T FM = dot3(ai, bi, ci, F, Fb, Z), FbM = dot3(bi, ai, mci, F, Fb, Z), ZM = dot3(zc0, zc1, di, F, Fb, Z);
-        if (store && s == 0) N::put(tabs(FbM), out_v, out_d, i * ostride);
+        if (store && s == 0) {
+            const T r = tabs(FbM);
+            N::put(r, out_v, out_d, i * ostride);
+            if (outT) outT[i] = N::val(r);
+        }
         const T Fup = shup<LPC>(FM), Fbdn = shdn<LPC>(FbM);
         F = (s == 0) ? FbM : Fup;
         Fb = (s == LPC - 1) ? N::c(0.0) : Fbdn;
@@ -258,9 +304,10 @@ __device__ void epg_general(double alpha_deg, double beta, double E1, double E2,
 
 template <int LPC, typename T>
 __device__ __forceinline__ void epg_curve(bool general, double alpha, double beta, double E1, double E2, int ETL,
-                                          int s, bool store, double *out_v, double *out_d, int64_t ostride) {
-    if (general) epg_general<LPC, T>(alpha, beta, E1, E2, ETL, s, store, out_v, out_d, ostride);
-    else epg_const<LPC, T>(alpha, E1, E2, ETL, s, store, out_v, out_d, ostride);
+                                          int s, bool store, gdbl *out_v, gdbl *out_d, int64_t ostride,
+                                          gdbl *outT) {
+    if (general) epg_general<LPC, T>(alpha, beta, E1, E2, ETL, s, store, out_v, out_d, ostride, outT);
+    else epg_const<LPC, T>(alpha, E1, E2, ETL, s, store, out_v, out_d, ostride, outT);
 }
 
 // Grid bases A_k and dA_k/dalpha for every flip angle of the search (EPGBasisSetEnsemble with
@@ -273,10 +320,23 @@ __global__ __launch_bounds__(64) void grid_basis_kernel(GridParams p) {
     const bool valid = j < p.n;
     const int jj = valid ? j : p.n - 1;
     const int64_t base = (int64_t)k * p.m * p.n + jj;
+    gdbl *outT = p.AsT ? (gdbl *)p.AsT + (int64_t)k * p.m * p.n + (int64_t)jj * p.m : nullptr;
     if (p.dAs)
-        epg_curve<LPC, Dual>(p.general, p.alphas[k], p.beta, p.E1, p.E2[jj], p.m, s, valid, p.As + base, p.dAs + base, p.n);
+        epg_curve<LPC, Dual>(p.general, p.alphas[k], p.beta, p.E1, p.E2[jj], p.m, s, valid, (gdbl *)p.As + base,
+                             (gdbl *)p.dAs + base, p.n, outT);
     else
-        epg_curve<LPC, double>(p.general, p.alphas[k], p.beta, p.E1, p.E2[jj], p.m, s, valid, p.As + base, nullptr, p.n);
+        epg_curve<LPC, double>(p.general, p.alphas[k], p.beta, p.E1, p.E2[jj], p.m, s, valid, (gdbl *)p.As + base,
+                               nullptr, p.n, outT);
+}
+
+// dA_k/dalpha with columns contiguous (K x n x m), for the coalesced row-lane gradient of the search.
+__global__ void transpose_kernel(const double *src, double *dst, int m, int n, int K) {
+    const int64_t total = (int64_t)K * m * n;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t k = t / ((int64_t)m * n), r = t % ((int64_t)m * n);
+        int j = (int)(r / m), i = (int)(r % m);
+        dst[t] = src[(k * m + i) * n + j];
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -293,21 +353,22 @@ __global__ void select_kernel(const double *image, int64_t V, double thr, int64_
 // ------------------------------------------------------------------------------------------------
 struct Wave {
     // LDS
-    double *A, *b, *zz, *x, *w, *xf, *b0, *cmu, *crn, *csemi;
-    int *idx, *diag, *stk;
+    ldbl *A, *b, *zz, *x, *w, *xf, *b0, *cmu, *crn, *csemi;
+    lint *idx, *diag, *stk;
     int lda, mw, m, n, L;
     // per-workgroup global scratch
-    double *A0s, *cx, *lc;
-    const double *A0;   // basis used by the regularization solves (A0s or the fixed basis)
+    gdbl *A0s, *A0T, *cx, *lc;
+    cgdbl *A0;   // basis used by the regularization solves (A0s or the fixed basis), row-major
+    cgdbl *A0t;  // the same basis with columns contiguous
     int ccur;
     unsigned long long flops, solves;
     int status;
 
-    __device__ __forceinline__ double &Aat(int i, int j) { return A[i * lda + j]; }
+    __device__ __forceinline__ ldbl &Aat(int i, int j) { return A[i * lda + j]; }
 
     // -------------------------------- Lawson-Hanson NNLS (src/NNLS.jl) --------------------------------
     // construct_apply_householder! (:259-336), 0-based: pivot row ip, column jp, rows [0, mr).
-    __device__ double construct_apply_householder(int ip, int jp, int mr) {
+    __device__ __forceinline__ double construct_apply_householder(int ip, int jp, int mr) {
         if (ip >= mr) return 0.0;
         double alpha = rfl(Aat(ip, jp));
         double part = 0.0;
@@ -358,15 +419,17 @@ struct Wave {
     }
 
     // apply_householder_dual! (:379-436): columns j > j1 (one per lane), rows [j1, m1).
-    __device__ void apply_householder_dual(double tau, int j1, int m1) {
+    __device__ __forceinline__ void apply_householder_dual(double tau, int j1, int m1) {
         if (j1 >= m1 - 1) return;
         const int j = L;
         if (j > j1 && j < n) {
             double sm = Aat(j1, j);
+#pragma unroll 8
             for (int i = j1 + 1; i < m1; ++i) sm = fma(Aat(i, j), Aat(i, j1), sm);
             sm *= -tau;
             Aat(j1, j) = Aat(j1, j) + sm;
             double wj = 0.0;
+#pragma unroll 8
             for (int i = j1 + 1; i < m1; ++i) {
                 double Aij = fma(sm, Aat(i, j1), Aat(i, j));
                 wj = fma(Aij, b[i], wj);
@@ -379,10 +442,11 @@ struct Wave {
     }
 
     // compute_dual! (:441-470): w_j = sum_{i >= j1} A_ij b_i for j >= j1, rows [j1, m1).
-    __device__ void compute_dual(int j1, int m1) {
+    __device__ __forceinline__ void compute_dual(int j1, int m1) {
         const int j = L;
         if (j >= j1 && j < n) {
             double sm = 0.0;
+#pragma unroll 8
             for (int i = j1; i < m1; ++i) sm = fma(Aat(i, j), b[i], sm);
             w[j] = sm;
         }
@@ -391,11 +455,19 @@ struct Wave {
     }
 
     // solve_triangular_system! (:506-524) on R = A[0:k, 0:k]; lane i holds z_i in a register.
-    __device__ double tri_solve(int k, double zr) {
+    __device__ __forceinline__ double tri_solve(int k, double zr) {
+        if (k <= 0) return zr;
+        // loads for column j are issued one iteration ahead of their use
+        double ajj_n = Aat(k - 1, k - 1), alj_n = (L < k - 1) ? Aat(L, k - 1) : 0.0;
         for (int j = k - 1; j >= 0; --j) {
+            const double ajj = rfl(ajj_n), alj = alj_n;
+            if (j > 0) {
+                ajj_n = Aat(j - 1, j - 1);
+                alj_n = (L < j - 1) ? Aat(L, j - 1) : 0.0;
+            }
             double zj = readlane(zr, j);
-            double zi = -zj / rfl(Aat(j, j));
-            if (L < j) zr = fma(Aat(L, j), zi, zr);
+            double zi = -zj / ajj;
+            if (L < j) zr = fma(alj, zi, zr);
             else if (L == j) zr = -zi;
         }
         flops += (unsigned long long)k * k;
@@ -404,7 +476,7 @@ struct Wave {
 
     // Remove coefficient imv from set P by Givens rotations and column shifts, then drop any further
     // nonpositive coefficients (:729-779 / :965-1015).
-    __device__ void nnls_delete(int &imv, int &nsetp, int mrows) {
+    __device__ __forceinline__ void nnls_delete(int &imv, int &nsetp, int mrows) {
         for (;;) {
             if (L == 0) x[idx[imv]] = 0.0;
             __syncthreads();
@@ -452,7 +524,7 @@ struct Wave {
 
     // unsafe_nnls! (:605-825) and its Tikhonov twin unsafe_nnls!(work, λ) (:827-1061). Workspace rows
     // [0, M) with M = m (plain) or m + n (Tikhonov rows appended lazily when a column first enters).
-    __device__ void nnls_main(bool tikh, double lam, int &nsetp_out, double &rnorm_out) {
+    __device__ __forceinline__ void nnls_main(bool tikh, double lam, int &nsetp_out, double &rnorm_out) {
         const int M = tikh ? m + n : m;
         int mr = m;  // current number of active rows (the reference's m)
         int nsetp = 0, iter = 0;
@@ -539,7 +611,7 @@ struct Wave {
 
     // NNLSProblem solve! with the reference warm start (src/lsqnonneg.jl:30-84), or the Tikhonov form
     // (:86-164) when tikh. A0g: global row-major m x n basis; right-hand side b0 in LDS.
-    __device__ void solve(const double *__restrict__ A0g, bool tikh, double mu, int &nsetp, double &rnorm) {
+    __device__ __forceinline__ void solve(cgdbl *__restrict__ A0g, bool tikh, double mu, int &nsetp, double &rnorm) {
         const int M = tikh ? m + n : m;
         double pd = 0.0;
         for (int i = L; i < m; i += kWave) { double a = A0g[i * n + n - 1]; pd += a * a; }
@@ -553,6 +625,7 @@ struct Wave {
         bool pos = false;
         if (L < n) {
             double wj = 0.0;
+#pragma unroll 8
             for (int i = 0; i < m; ++i) {
                 double a = A0g[i * n + L];
                 wj += a * zz[i];
@@ -579,11 +652,11 @@ struct Wave {
     // ------------------------- Tikhonov cache (src/lsqnonneg.jl:392-444) -------------------------
     // A hit needs exactly equal mu; re-solving the same mu reproduces the same bits, so the cache is a pure
     // memo. Entries keep mu, rnorm, ||x||^2 and x (global scratch).
-    __device__ void cache_reset() {
+    __device__ __forceinline__ void cache_reset() {
         if (L < kTikhCache) cmu[L] = NAN;
         __syncthreads();
     }
-    __device__ int cache_solve(double mu) {
+    __device__ __forceinline__ int cache_solve(double mu) {
         int hit = -1, emp = -1;
         for (int i = 0; i < kTikhCache; ++i) {
             double c = rfl(cmu[i]);
@@ -603,17 +676,17 @@ struct Wave {
         __syncthreads();
         return slot;
     }
-    __device__ double cur_mu() { return rfl(cmu[ccur]); }
-    __device__ double cur_semi() { return rfl(csemi[ccur]); }
-    __device__ double cur_resnorm_sq() {
+    __device__ __forceinline__ double cur_mu() { return rfl(cmu[ccur]); }
+    __device__ __forceinline__ double cur_semi() { return rfl(csemi[ccur]); }
+    __device__ __forceinline__ double cur_resnorm_sq() {
         double mu = cur_mu(), rn = rfl(crn[ccur]);
         return jl_max(rn * rn - (mu * mu) * cur_semi(), 0.0);
     }
-    __device__ void take_cur_solution() {
+    __device__ __forceinline__ void take_cur_solution() {
         if (L < n) xf[L] = cx[ccur * n + L];
         __syncthreads();
     }
-    __device__ double plain_solve_to_xf() {
+    __device__ __forceinline__ double plain_solve_to_xf() {
         int ns;
         double rn;
         solve(A0, false, 0.0, ns, rn);
@@ -621,7 +694,7 @@ struct Wave {
         __syncthreads();
         return rn;
     }
-    __device__ double plain_solve_res2(int &ns) {
+    __device__ __forceinline__ double plain_solve_res2(int &ns) {
         double rn;
         solve(A0, false, 0.0, ns, rn);
         return rn * rn;
@@ -630,7 +703,7 @@ struct Wave {
 
 // --------------------------------- scalar optimizers (src/optimization.jl) ---------------------------------
 // bracket_root_monotonic (:191-219)
-template <class F> __device__ void bracket_root_monotonic(F &&f, double a, double delta, double dilate, double mono,
+template <class F> __device__ __forceinline__ void bracket_root_monotonic(F &&f, double a, double delta, double dilate, double mono,
                                                           int maxiters, double out[4]) {
     double fa = f(a);
     if (!isfinite(fa)) { out[0] = a; out[1] = a; out[2] = NAN; out[3] = NAN; return; }
@@ -655,7 +728,7 @@ template <class F> __device__ void bracket_root_monotonic(F &&f, double a, doubl
 }
 
 // brent_root (:71-128)
-template <class F> __device__ void brent_root(F &&f, double x0, double x1, double fx0, double fx1, double xatol,
+template <class F> __device__ __forceinline__ void brent_root(F &&f, double x0, double x1, double fx0, double fx1, double xatol,
                                               double xrtol, double ftol, int maxiters, double &xo, double &fo) {
     if (fx0 == 0) { xo = x0; fo = fx0; return; }
     if (fx1 == 0) { xo = x1; fo = fx1; return; }
@@ -693,7 +766,7 @@ template <class F> __device__ void brent_root(F &&f, double x0, double x1, doubl
 }
 
 // brent_minimize (:319-413)
-template <class F> __device__ void brent_minimize(F &&f, double x1, double x2, double xrtol, double xatol, int maxiters,
+template <class F> __device__ __forceinline__ void brent_minimize(F &&f, double x1, double x2, double xrtol, double xatol, int maxiters,
                                                   double &xo, double &yo) {
     const double alpha = 2 - kGolden;
     double x = x1 + alpha * (x2 - x1);
@@ -746,7 +819,7 @@ template <class F> __device__ void brent_minimize(F &&f, double x1, double x2, d
 
 // ------------------------------------ regularization (src/lsqnonneg.jl) ------------------------------------
 // lsqnonneg_chi2! with method :brent (:504-593)
-__device__ void reg_chi2(Wave &W, double chi2_target, double &mu_out, double &chi2_out) {
+__device__ __forceinline__ void reg_chi2(Wave &W, double chi2_target, double &mu_out, double &chi2_out) {
     int ns;
     double res2_min = W.plain_solve_res2(ns);
     if (res2_min == 0 || ns == 0) {
@@ -786,7 +859,7 @@ __device__ void reg_chi2(Wave &W, double chi2_target, double &mu_out, double &ch
 }
 
 // lsqnonneg_mdp! (:700-747)
-__device__ void reg_mdp(Wave &W, double delta, double &mu_out, double &chi2_out) {
+__device__ __forceinline__ void reg_mdp(Wave &W, double delta, double &mu_out, double &chi2_out) {
     int ns;
     double res2_min = W.plain_solve_res2(ns);
     if (W.L < W.n) W.xf[W.L] = W.x[W.L];
@@ -834,13 +907,13 @@ __device__ void reg_mdp(Wave &W, double delta, double &mu_out, double &chi2_out)
 // calls LAPACK dgesdd (src/utils.jl:96-134); here: Gram matrix of the smaller side, Householder
 // tridiagonalization, then Sturm-sequence bisection, one eigenvalue per lane. Absolute accuracy is
 // eps * ||A||^2, far below the mu^2 >= exp(-16) scale at which gcv_dof uses them.
-__device__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
+__device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
     const int m = W.m, n = W.n, L = W.L, lda = W.lda;
     if (L < n)
         for (int i = 0; i < m; ++i) W.Aat(i, L) = W.A0[i * n + L];
     __syncthreads();
     N = min(m, n);
-    double *G = W.A + (size_t)m * lda;  // N x N at rows [m, m + N)
+    ldbl *G = W.A + (size_t)m * lda;  // N x N at rows [m, m + N)
     if (L < N) {
         if (m >= n) {
             for (int c = 0; c < N; ++c) {
@@ -858,7 +931,7 @@ __device__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
     }
     W.flops += 2ull * N * N * (m >= n ? m : n);
     __syncthreads();
-    double *dv = W.b, *ev = W.b + N, *vv = W.zz, *qv = W.zz + N;
+    ldbl *dv = W.b, *ev = W.b + N, *vv = W.zz, *qv = W.zz + N;
     for (int k = 0; k < N - 2; ++k) {
         double xr = (L > k && L < N) ? G[L * lda + k] : 0.0;
         double nrm = sqrt(wsum(xr * xr));
@@ -934,7 +1007,7 @@ __device__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
 }
 
 // lsqnonneg_gcv! with method :brent (:1136-1205)
-__device__ void reg_gcv(Wave &W, double &mu_out, double &chi2_out) {
+__device__ __forceinline__ void reg_gcv(Wave &W, double &mu_out, double &chi2_out) {
     double lam;
     int N;
     gram_eigs(W, lam, N);
@@ -968,14 +1041,14 @@ __device__ void reg_gcv(Wave &W, double &mu_out, double &chi2_out) {
 // lane-parallel reductions that reproduce the reference's sequential first-match / first-max rules.
 struct LCurve {
     Wave &W;
-    double *fcx, *fcP0, *fcP1;
-    double *pcx, *pcP0, *pcP1, *pcC;
-    double *sx[4], *sP[4][2];
+    gdbl *fcx, *fcP0, *fcP1;
+    gdbl *pcx, *pcP0, *pcP1, *pcC;
+    gdbl *sx[4], *sP[4][2];
     int nfc, npc, nsc;
     bool overflow;
 
-    __device__ LCurve(Wave &w) : W(w), nfc(0), npc(0), nsc(0), overflow(false) {
-        double *p = w.lc;
+    __device__ __forceinline__ LCurve(Wave &w) : W(w), nfc(0), npc(0), nsc(0), overflow(false) {
+        gdbl *p = w.lc;
         fcx = p; fcP0 = p + kLcurveCap; fcP1 = p + 2 * kLcurveCap;
         p += 3 * kLcurveCap;
         pcx = p; pcP0 = p + kLcurveCap; pcP1 = p + 2 * kLcurveCap; pcC = p + 3 * kLcurveCap;
@@ -984,7 +1057,7 @@ struct LCurve {
         for (int i = 0; i < 4; ++i)
             for (int c = 0; c < 2; ++c) { sP[i][c] = p; p += kLcurveCap; }
     }
-    __device__ int find(const double *keys, int cnt, double x) {
+    __device__ __forceinline__ int find(const gdbl *keys, int cnt, double x) {
         int first = INT_MAX;
         for (int base = 0; base < cnt; base += kWave) {
             int k = base + W.L;
@@ -994,7 +1067,7 @@ struct LCurve {
         }
         return first == INT_MAX ? -1 : first;
     }
-    __device__ void eval(double x, double &P0, double &P1) {
+    __device__ __forceinline__ void eval(double x, double &P0, double &P1) {
         int i = find(fcx, nfc, x);
         if (i >= 0) { P0 = rfl(fcP0[i]); P1 = rfl(fcP1[i]); return; }
         W.cache_solve(exp(x));
@@ -1005,7 +1078,7 @@ struct LCurve {
         nfc++;
         __syncthreads();
     }
-    __device__ void pc_set(double x, double P0, double P1, double C) {
+    __device__ __forceinline__ void pc_set(double x, double P0, double P1, double C) {
         int i = find(pcx, npc, x);
         if (i < 0) {
             if (npc >= kLcurveCap) { overflow = true; return; }
@@ -1015,7 +1088,7 @@ struct LCurve {
         if (W.L == 0) { pcP0[i] = P0; pcP1[i] = P1; pcC[i] = C; }
         __syncthreads();
     }
-    __device__ int pc_argmax() {
+    __device__ __forceinline__ int pc_argmax() {
         // findmax with isless (NaN largest), first occurrence
         double bc = -INFINITY;
         int bi = INT_MAX;
@@ -1044,7 +1117,7 @@ struct LCurve {
         }
         return rfl(bi);
     }
-    __device__ void update_curvature(const double xs[4], const double Ps[4][2], const double Ptl[2], const double Pbr[2],
+    __device__ __forceinline__ void update_curvature(const double xs[4], const double Ps[4][2], const double Ptl[2], const double Pbr[2],
                                      double Ctol) {
         for (int i = 0; i < 4; ++i) {
             double x = xs[i], C = -INFINITY;
@@ -1084,7 +1157,7 @@ struct LCurve {
             pc_set(x, P[0], P[1], C);
         }
     }
-    __device__ double corner(double xlow, double xhigh, double xtol, double Ptol, double Ctol) {
+    __device__ __forceinline__ double corner(double xlow, double xhigh, double xtol, double Ptol, double Ctol) {
         double xs[4], Ps[4][2];
         xs[0] = xlow;
         xs[3] = xhigh;
@@ -1162,7 +1235,7 @@ struct LCurve {
     }
 };
 
-__device__ void reg_lcurve(Wave &W, double &mu_out, double &chi2_out) {
+__device__ __forceinline__ void reg_lcurve(Wave &W, double &mu_out, double &chi2_out) {
     W.cache_reset();
     LCurve lc(W);
     double logmu = lc.corner(-8.0, 2.0, 1e-4, 1e-4, 1e-4);
@@ -1185,7 +1258,7 @@ __device__ __forceinline__ double evalpoly4(double t, double c0, double c1, doub
     return fma(t, fma(t, fma(t, c3, c2), c1), c0);
 }
 // CubicHermiteInterpolator + minimize (src/splines.jl:62-110)
-__device__ void hermite_minimize(double a, double b, double u0, double u1, double m0, double m1, double &xo, double &uo) {
+__device__ __forceinline__ void hermite_minimize(double a, double b, double u0, double u1, double m0, double m1, double &xo, double &uo) {
     double r = (b - a) / 2;
     m0 = r * m0;
     m1 = r * m1;
@@ -1211,14 +1284,15 @@ __device__ void hermite_minimize(double a, double b, double u0, double u1, doubl
 
 struct FlipSearch {
     Wave &W;
-    const double *As, *dAs;
+    cgdbl *As, *AsT, *dAsT;
     int K, numeval, mineval;
     unsigned long long seen;
     double al, u, du;  // lane k: grid angle, loss, gradient
 
-    __device__ void eval(int k) {
+    __device__ __forceinline__ void eval(int k) {
         const int m = W.m, n = W.n;
-        const double *A = As + (size_t)k * m * n, *dA = dAs + (size_t)k * m * n;
+        cgdbl *A = As + (size_t)k * m * n;
+        cgdbl *AT = AsT + (size_t)k * m * n, *dAT = dAsT + (size_t)k * m * n;
         int ns;
         double rn;
         W.solve(A, false, 0.0, ns, rn);
@@ -1228,7 +1302,7 @@ struct FlipSearch {
             double ax = 0.0, dax = 0.0;
             for (int j = 0; j < n; ++j) {
                 double xj = W.x[j];
-                if (xj > 0) { ax = fma(xj, A[i * n + j], ax); dax = fma(xj, dA[i * n + j], dax); }
+                if (xj > 0) { ax = fma(xj, AT[j * m + i], ax); dax = fma(xj, dAT[j * m + i], dax); }
             }
             ax -= W.b0[i];
             part += dax * ax;
@@ -1238,7 +1312,7 @@ struct FlipSearch {
         if (W.L == k) { u = ell; du = 2 * dt; }
     }
     // update!(surr, state, I; maxeval) (:736-744)
-    __device__ bool update(int k) {
+    __device__ __forceinline__ bool update(int k) {
         if (numeval >= K) return true;
         if ((seen >> k) & 1ull) return false;
         eval(k);
@@ -1246,9 +1320,9 @@ struct FlipSearch {
         numeval += 1;
         return numeval >= K;
     }
-    __device__ bool box_evaluated(int lo, int hi) { return ((seen >> lo) & 1ull) && ((seen >> hi) & 1ull); }
+    __device__ __forceinline__ bool box_evaluated(int lo, int hi) { return ((seen >> lo) & 1ull) && ((seen >> hi) & 1ull); }
     // evaluate_box! (:802-815); natural corner order, or sorted by distance to x
-    __device__ void evaluate_box(int lo, int hi, bool sorted, double x) {
+    __device__ __forceinline__ void evaluate_box(int lo, int hi, bool sorted, double x) {
         int c0 = lo, c1 = hi;
         if (sorted) {
             double alo = readlane(al, lo), ahi = readlane(al, hi);
@@ -1261,7 +1335,7 @@ struct FlipSearch {
         update(c1);
     }
     // suggest_point (:544-566): one Hermite segment per evaluated lane, earliest minimum wins.
-    __device__ void suggest(double &xo, double &uo) {
+    __device__ __forceinline__ void suggest(double &xo, double &uo) {
         const int k = W.L;
         int f = first_set(seen);
         double uf = readlane(u, f), xf = readlane(al, f);
@@ -1283,7 +1357,7 @@ struct FlipSearch {
         xo = readlane(xs, ord);
         uo = ys;
     }
-    __device__ double run() {
+    __device__ __forceinline__ double run() {
         seen = 0ull;
         numeval = 0;
         // initialize! (:716-734): depth-limited pre-order bisection, explicit stack
@@ -1334,8 +1408,8 @@ struct FlipSearch {
 };
 
 // Basis at one flip angle: n EPG curves, two per wave (32-lane groups) when the echo window fits.
-__device__ void voxel_basis(Wave &W, bool general, double alpha, double beta, double E1, const double *E2,
-                            double *A0out) {
+__device__ __forceinline__ void voxel_basis(Wave &W, bool general, double alpha, double beta, double E1, cgdbl *E2,
+                            gdbl *A0out, gdbl *A0T) {
     const int m = W.m, n = W.n, L = W.L;
     if (m / 2 + 2 <= 32) {
         const int h = L >> 5, s = L & 31;
@@ -1343,10 +1417,12 @@ __device__ void voxel_basis(Wave &W, bool general, double alpha, double beta, do
             int j = jp + h;
             bool valid = j < n;
             int jj = valid ? j : n - 1;
-            epg_curve<32, double>(general, alpha, beta, E1, E2[jj], m, s, valid, A0out + jj, nullptr, n);
+            epg_curve<32, double>(general, alpha, beta, E1, E2[jj], m, s, valid, A0out + jj, nullptr, n,
+                                  A0T + (int64_t)jj * m);
         }
     } else {
-        for (int j = 0; j < n; ++j) epg_curve<64, double>(general, alpha, beta, E1, E2[j], m, L, true, A0out + j, nullptr, n);
+        for (int j = 0; j < n; ++j)
+            epg_curve<64, double>(general, alpha, beta, E1, E2[j], m, L, true, A0out + j, nullptr, n, A0T + (int64_t)j * m);
     }
     // EPG op count of the reference's split recursion: ~13 flops per retained state per echo
     W.flops += (unsigned long long)n * 13ull * ((unsigned long long)m * m / 4 + m);
@@ -1363,7 +1439,7 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
     W.n = p.n;
     W.lda = p.lda;
     W.mw = p.mw;
-    double *s = smem;
+    ldbl *s = (ldbl *)smem;
     W.A = s; s += (size_t)p.mw * p.lda;
     W.b = s; s += p.mw;
     W.zz = s; s += p.mw;
@@ -1374,13 +1450,14 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
     W.cmu = s; s += kTikhCache;
     W.crn = s; s += kTikhCache;
     W.csemi = s; s += kTikhCache;
-    int *si = reinterpret_cast<int *>(s);
+    lint *si = (lint *)s;
     W.idx = si; si += kMaxT2;
     W.diag = si; si += kMaxT2;
     W.stk = si;
-    double *slot = p.scratch + (int64_t)blockIdx.x * p.slot_stride;
+    gdbl *slot = (gdbl *)p.scratch + (int64_t)blockIdx.x * p.slot_stride;
     W.A0s = slot;
-    W.cx = slot + (int64_t)p.m * p.n;
+    W.A0T = slot + (int64_t)p.m * p.n;
+    W.cx = W.A0T + (int64_t)p.m * p.n;
     W.lc = W.cx + (int64_t)kTikhCache * p.n;
     W.flops = 0;
     W.solves = 0;
@@ -1393,11 +1470,11 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
         if (L == 0) vi = atomicAdd(p.counter, 1u);
         vi = (unsigned)rfl((int)__shfl((int)vi, 0, kWave));
         if ((int64_t)vi >= p.nvox) break;
-        const int64_t I = p.vox[vi];
+        const int64_t I = ((cgi64 *)p.vox)[vi];
         // voxelwise_T2_distribution! (:201-238): normalize by the max echo
         double mx = 0.0;
         for (int i = L; i < m; i += kWave) {
-            double bi = p.image[I + V * i];
+            double bi = ((cgdbl *)p.image)[I + V * i];
             mx = bi > mx ? bi : mx;
             W.b0[i] = bi;
         }
@@ -1409,19 +1486,21 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
         double alpha_v;
         if (p.mode == kModeFixed) {
             alpha_v = p.setflip;
-            W.A0 = p.Afixed;
+            W.A0 = (cgdbl *)p.Afixed;
+            W.A0t = (cgdbl *)p.AfixedT;
         } else {
             if (p.mode == kModeB1) {
                 alpha_v = p.alpha_in[I];
             } else {
-                FlipSearch fs{W, p.As, p.dAs, p.K, 0, p.mineval, 0ull, 0.0, 0.0, 0.0};
+                FlipSearch fs{W, (cgdbl *)p.As, (cgdbl *)p.AsT, (cgdbl *)p.dAsT, p.K, 0, p.mineval, 0ull, 0.0, 0.0, 0.0};
                 fs.al = (L < p.K) ? p.alphas[L] : p.alphas[p.K - 1];
                 fs.u = NAN;
                 fs.du = NAN;
                 alpha_v = fs.run();
             }
-            voxel_basis(W, p.general != 0, alpha_v, p.beta, p.E1, p.E2, W.A0s);
+            voxel_basis(W, p.general != 0, alpha_v, p.beta, p.E1, (cgdbl *)p.E2, W.A0s, W.A0T);
             W.A0 = W.A0s;
+            W.A0t = W.A0T;
         }
         // T2_distribution! (src/T2mapSEcorr.jl:475-505)
         double muv = 0.0, chi2v = 1.0;
@@ -1443,7 +1522,7 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
         double pr2 = 0.0, prs = 0.0;
         for (int i = L; i < m; i += kWave) {
             double fit = 0.0;
-            for (int j = 0; j < n; ++j) fit += W.A0[i * n + j] * W.xf[j];
+            for (int j = 0; j < n; ++j) fit += W.A0t[j * m + i] * W.xf[j];
             double bi = W.b0[i] * max_signal;
             double res = fit - bi;
             if (i == L) res0 = res; else res1 = res;
@@ -1458,7 +1537,7 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
         if (L < m) pv += (res0 - mean_res) * (res0 - mean_res);
         if (L + kWave < m) pv += (res1 - mean_res) * (res1 - mean_res);
         double sd_res = sqrt(wsum(pv) / (m - 1));
-        double lt = (L < n) ? p.logT2[L] : 0.0;
+        double lt = (L < n) ? ((cgdbl *)p.logT2)[L] : 0.0;
         double log_ggm = wsum(xs * lt) / sum_dist;
         double dv = (L < n) ? (lt - log_ggm) * (lt - log_ggm) * xs : 0.0;
         double log1p_gva = wsum(dv) / sum_dist;
@@ -1520,7 +1599,7 @@ size_t t2map_lds_bytes(int m, int n, int lda, int mw) {
 }
 
 int64_t t2map_slot_doubles(int m, int n) {
-    int64_t s = (int64_t)m * n + (int64_t)kTikhCache * n + 19 * (int64_t)kLcurveCap;
+    int64_t s = 2 * (int64_t)m * n + (int64_t)kTikhCache * n + 19 * (int64_t)kLcurveCap;
     return (s + 31) / 32 * 32;
 }
 
@@ -1532,6 +1611,13 @@ hipError_t launch_grid_basis(const GridParams &p, hipStream_t s) {
         dim3 grid(p.K, p.n);
         hipLaunchKernelGGL(grid_basis_kernel<64>, grid, dim3(64), 0, s, p);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose(const double *src, double *dst, int m, int n, int K, hipStream_t s) {
+    int64_t total = (int64_t)K * m * n;
+    int blocks = (int)std::min<int64_t>((total + 255) / 256, 2048);
+    hipLaunchKernelGGL(transpose_kernel, dim3(blocks), dim3(256), 0, s, src, dst, m, n, K);
     return hipGetLastError();
 }
 
