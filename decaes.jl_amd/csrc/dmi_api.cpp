@@ -108,20 +108,23 @@ void logrange(double a, double b, int n, double *out) {
     out[n - 1] = b;
 }
 
-// Inverse error function: Newton-Halley refinement of Winitzki's (2008) closed-form start on libm erf.
+// Inverse error function: Winitzki's (2008) closed-form start refined by Halley steps, on erf near zero and on
+// erfc against 1 - |y| (exact for |y| >= 1/2) in the tails.
 double erfinv(double y) {
     if (y <= -1.0) return y == -1.0 ? -INFINITY : NAN;
     if (y >= 1.0) return y == 1.0 ? INFINITY : NAN;
     if (y == 0.0) return y;
     const double a = 0.147, pi = 3.141592653589793;
+    const double ay = std::fabs(y), q = 1.0 - ay;
     double l = std::log1p(-y * y), t = 2.0 / (pi * a) + 0.5 * l;
-    double x = std::copysign(std::sqrt(std::sqrt(t * t - l / a) - t), y);
+    double x = std::sqrt(std::sqrt(t * t - l / a) - t);
     for (int it = 0; it < 5; ++it) {
-        double f = std::erf(x) - y, fp = 2.0 / std::sqrt(pi) * std::exp(-x * x);
+        double f = ay < 0.5 ? std::erf(x) - ay : q - std::erfc(x);
+        double fp = 2.0 / std::sqrt(pi) * std::exp(-x * x);
         double s = f / fp;
         x -= s / (1.0 + x * s);
     }
-    return x;
+    return std::copysign(x, y);
 }
 
 // T2mapOptions @assert checks, in the reference's order and wording (src/types.jl:28-84).

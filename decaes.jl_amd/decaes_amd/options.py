@@ -27,7 +27,7 @@ def _linrange(a: float, b: float, n: int) -> np.ndarray:
 
 def _logrange(a: float, b: float, n: int) -> np.ndarray:
     """logrange (src/utils.jl:7): exp of a linear range in log space, exact endpoints."""
-    r = np.exp(_linrange(math.log(a), math.log(b), n))
+    r = np.array([math.exp(v) for v in _linrange(math.log(a), math.log(b), n)])  # libm exp, as the library
     r[0], r[-1] = a, b
     return r
 

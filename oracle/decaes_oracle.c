@@ -103,21 +103,24 @@ double orc_cosd(double x) {
     return (double)cosl(deg2rad_extl(360.0 - rx));
 }
 
-/* erfinv: Winitzki's closed-form start refined by Halley steps on libm erf (own method, not a copy). */
+/* erfinv: Winitzki's closed-form start refined by Halley steps on libm erf, or on erfc against 1 - |y|
+ * (exact for |y| >= 1/2) in the tails where erf(x) - y would cancel (own method, not a copy). */
 double orc_erfinv(double y) {
     if (y <= -1.0) return y == -1.0 ? -INFINITY : NAN;
     if (y >= 1.0) return y == 1.0 ? INFINITY : NAN;
     if (y == 0.0) return y;
     const double a = 0.147, pi = 3.141592653589793;
+    double ay = fabs(y);
     double l = log1p(-y * y), t = 2.0 / (pi * a) + 0.5 * l;
-    double x = copysign(sqrt(sqrt(t * t - l / a) - t), y);
-    for (int it = 0; it < 4; ++it) {
-        double err = erf(x) - y;
+    double x = sqrt(sqrt(t * t - l / a) - t);
+    double q = 1.0 - ay;
+    for (int it = 0; it < 5; ++it) {
+        double err = ay < 0.5 ? erf(x) - ay : q - erfc(x);
         double d = 2.0 / sqrt(pi) * exp(-x * x);
         double step = err / d;
         x -= step / (1.0 + x * step); /* Halley: f''/f' = -2x */
     }
-    return x;
+    return copysign(x, y);
 }
 
 /* ------------------------------------------------------------------------------------------------
