@@ -5,6 +5,7 @@ NNLS, maps) followed by T2partSEcorr on its distribution, with the image and all
 value = processed (above-threshold) voxels per second. Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3] [--no-cpu-baseline]
+    python bench.py --copy-probe BYTES    # plain device copy of known size, the HBM-counter calibration run
 """
 from __future__ import annotations
 
@@ -31,6 +32,20 @@ WORKLOADS = {
 }
 SPWIN = (None, 40e-3)  # short-peak window upper edge (lower edge = T2Range[0])
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
+METRIC = "voxels/s (T2map+T2part) at MWF err\u22641e-3 vs plain; served p50/p99 latency"
+TRAFFIC_FILE = {"c2": ROOT / "profiles" / "t2map_kernel_gcv_240x240x48_hbm_traffic.md"}
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per t2map_kernel launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes (calibrated
+    against a plain 1 GiB copy), or None when no such measurement exists for the workload."""
+    import re
+
+    f = TRAFFIC_FILE.get(workload)
+    if f is None or not f.exists():
+        return None
+    m = re.search(rf"^traffic_bytes_per_launch\[{workload}\]: (\d+)", f.read_text(), re.M)
+    return int(m.group(1)) if m else None
 
 
 def log(msg):
@@ -65,6 +80,21 @@ def cpu_baseline(img, opts_kw, seconds: float):
                 sample=f"{done} in-mask voxels drawn (seed 1234) from the same volume, {dt:.1f} s of CPU work")
 
 
+def copy_probe(nbytes: int):
+    """Three plain device-to-device copies of `nbytes`: the known-size run that scales FETCH_SIZE/WRITE_SIZE."""
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / 3
+    print(json.dumps({"copy_bytes": a.numel() * 4, "GBps_read_plus_write": 2 * a.numel() * 4 / dt / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,7 +103,10 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); voxels shard across GPUs with no exchange "
               "step, so a multi-GPU figure is N independent single-GPU runs.", file=sys.stderr)
@@ -128,7 +161,7 @@ def main():
     gdn = out["gdn"].cpu().numpy()
     finite = int(np.isfinite(gdn).sum())
     res = {
-        "metric": "voxels/s (T2map+T2part)",
+        "metric": METRIC,
         "value": nvox / (dt / args.steps),
         "unit": "voxels/s",
         "n_gpus": 1,
@@ -147,7 +180,7 @@ def main():
                    "nnls_solves_per_voxel": stats[0]["nnls_solves"] / max(nvox, 1),
                    "fp64_flops_per_voxel": flops / max(nvox, 1)},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(args.workload),
                      "kernel": f"t2map_kernel<{dmi.api._lib.REG[w['Reg']]}>", "kernel_ms": kernel_ms,
                      "flops_per_launch": flops},
     }
