@@ -84,7 +84,7 @@ typedef struct {
 typedef struct {
     int64_t voxels;        /* voxels above Threshold that were processed */
     int64_t nnls_solves;   /* Lawson-Hanson solves (flip search + regularization) */
-    int64_t fp64_flops;    /* algorithmic FP64 flops (formula table in DESIGN.md) */
+    int64_t fp64_flops;    /* algorithmic FP64 flops: per-step counts of EPG, Householder, Givens, solves, SVD */
     int64_t fp32_flops;    /* algorithmic FP32 flops (mixed precision EPG) */
     double kernel_ms;      /* device time of the voxel kernel, HIP events */
     double total_ms;       /* device time of the whole call */
