@@ -257,7 +257,7 @@ int run_t2map(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, const 
     p.m = m; p.n = n; p.K = fixed ? 1 : o->nRefAngles; p.mineval = o->nRefAnglesMin; p.reg = o->Reg;
     p.mode = b1 ? dmi::kModeB1 : (fixed ? dmi::kModeFixed : dmi::kModeSearch);
     p.general = o->RefConAngle != 180.0;
-    p.lda = (n % 2 == 0) ? n + 1 : n;
+    p.lda = dmi::t2map_lda(m, n);
     p.mw = m + n;
     p.save_resnorm = o->SaveResidualNorm; p.save_decaycurve = o->SaveDecayCurve;
     p.save_regparam = o->SaveRegParam; p.save_basis = o->SaveNNLSBasis;

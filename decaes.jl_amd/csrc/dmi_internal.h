@@ -34,7 +34,7 @@ struct GridParams {
 
 struct T2mapParams {
     int m, n, K, mineval, reg, mode, general;
-    int lda, mw;          // LDS workspace: mw = m + n rows, leading dimension lda
+    int lda, mw;          // LDS workspace: mw = m + n rows, column-major with column stride lda = t2map_lda(m, n)
     int save_resnorm, save_decaycurve, save_regparam, save_basis;
     int64_t V;            // voxels in the volume
     int64_t nvox;         // voxels above threshold
@@ -68,6 +68,7 @@ struct T2partParams {
     double *sfr, *sgm, *mfr, *mgm;
 };
 
+int t2map_lda(int m, int n);
 size_t t2map_lds_bytes(int m, int n, int lda, int mw);
 int64_t t2map_slot_doubles(int m, int n);
 

@@ -30,6 +30,9 @@ typedef __attribute__((address_space(3))) int lint;
 typedef __attribute__((address_space(1))) double gdbl;
 typedef const __attribute__((address_space(1))) double cgdbl;
 typedef const __attribute__((address_space(1))) int64_t cgi64;
+typedef double dv2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(3))) dv2 cldv2;
+typedef __attribute__((address_space(3))) dv2 ldv2;
 
 constexpr double kDeg2Rad = 3.141592653589793 / 180.0;  // Julia deg2rad: x * (Float64(pi)/180)
 constexpr double kPi180Hi = 0.017453292519943295;      // pi/180 as a double-double
@@ -340,6 +343,49 @@ __global__ void transpose_kernel(const double *src, double *dst, int m, int n, i
 }
 
 // ------------------------------------------------------------------------------------------------
+// Phase cycle counters (built only with -DDMI_PHASE_PROFILE, into a separate library): per-workgroup LDS
+// accumulators of shader-clock cycles and call counts per phase, summed into g_phase at wave exit.
+// ------------------------------------------------------------------------------------------------
+enum Phase : int {
+    kPhConstruct = 1, kPhApply = 2, kPhTriSolve = 3, kPhDelete = 4, kPhDual = 5, kPhInit = 6, kPhNnls = 7,
+    kPhGram = 8, kPhTridiag = 9, kPhFlipSearch = 10, kPhVoxelBasis = 11, kPhSturm = 12, kPhGramEigs = 13,
+    kPhRegTotal = 14, kPhVoxel = 15,
+    kPhCount = 16
+};
+#ifdef DMI_PHASE_PROFILE
+__device__ unsigned long long g_phase[2 * kPhCount];
+__shared__ unsigned long long s_phase[2 * kPhCount];
+#define DMI_PT0() unsigned long long _pt0 = clock64()
+// DMI_LAP(k): charge the cycles since the last DMI_PT0 / DMI_LAP to phase k and restart the timer
+#define DMI_LAP(k)                                                 \
+    do {                                                           \
+        const unsigned long long _now = clock64();                 \
+        if (lane_id() == 0) {                                      \
+            s_phase[k] += _now - _pt0;                             \
+            s_phase[kPhCount + (k)] += 1;                          \
+        }                                                          \
+        _pt0 = _now;                                               \
+    } while (0)
+#define DMI_PT1(k)                                                 \
+    do {                                                           \
+        if (lane_id() == 0) {                                      \
+            s_phase[k] += clock64() - _pt0;                        \
+            s_phase[kPhCount + (k)] += 1;                          \
+        }                                                          \
+    } while (0)
+#define DMI_PHASE_BEGIN() \
+    do { if (lane_id() == 0) for (int _k = 0; _k < 2 * kPhCount; ++_k) s_phase[_k] = 0; } while (0)
+#define DMI_PHASE_END() \
+    do { if (lane_id() == 0) for (int _k = 0; _k < 2 * kPhCount; ++_k) atomicAdd(&g_phase[_k], s_phase[_k]); } while (0)
+#else
+#define DMI_PT0() do {} while (0)
+#define DMI_PT1(k) do {} while (0)
+#define DMI_LAP(k) do {} while (0)
+#define DMI_PHASE_BEGIN() do {} while (0)
+#define DMI_PHASE_END() do {} while (0)
+#endif
+
+// ------------------------------------------------------------------------------------------------
 // Voxel selection: indices with image[I, 1] > Threshold (src/T2mapSEcorr.jl:177). Order is irrelevant:
 // voxels are independent and results are written at their own index.
 // ------------------------------------------------------------------------------------------------
@@ -352,7 +398,9 @@ __global__ void select_kernel(const double *image, int64_t V, double thr, int64_
 // Per-voxel wave state
 // ------------------------------------------------------------------------------------------------
 struct Wave {
-    // LDS
+    // LDS. A is column-major with column stride lda (even, lda = 2 mod 4, lda >= mw rounded up to even):
+    // a lane that owns a column streams it with 16-byte loads, and 16-byte loads of 16 consecutive lanes
+    // at stride lda hit distinct banks. Every LDS double array starts 16-byte aligned.
     ldbl *A, *b, *zz, *x, *w, *xf, *b0, *cmu, *crn, *csemi;
     lint *idx, *diag, *stk;
     int lda, mw, m, n, L;
@@ -364,15 +412,29 @@ struct Wave {
     unsigned long long flops, solves;
     int status;
 
-    __device__ __forceinline__ ldbl &Aat(int i, int j) { return A[i * lda + j]; }
+    __device__ __forceinline__ ldbl &Aat(int i, int j) { return A[j * lda + i]; }
 
     // -------------------------------- Lawson-Hanson NNLS (src/NNLS.jl) --------------------------------
-    // construct_apply_householder! (:259-336), 0-based: pivot row ip, column jp, rows [0, mr).
+    // construct_apply_householder! (:259-336), 0-based: pivot row ip, column jp, rows [0, mr). Lane L holds
+    // rows L, L + 64 and L + 128 of column jp (mr <= m + n <= 188) in registers; each element is divided by
+    // alpha once and reused for the right-hand-side update and the column write.
     __device__ __forceinline__ double construct_apply_householder(int ip, int jp, int mr) {
         if (ip >= mr) return 0.0;
-        double alpha = rfl(Aat(ip, jp));
+        const ldbl *cp = A + jp * lda;
+        double c[3], bv[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int i = L + k * kWave;
+            c[k] = (i < mr) ? cp[i] : 0.0;
+            bv[k] = (i < mr) ? b[i] : 0.0;
+        }
+        double alpha = rfl(cp[ip]);
         double part = 0.0;
-        for (int i = ip + L; i < mr; i += kWave) { double a = Aat(i, jp); part = fma(a, a, part); }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int i = L + k * kWave;
+            if (i >= ip && i < mr) part = fma(c[k], c[k], part);
+        }
         double xnorm = sqrt(wsum(part));
         flops += 2ull * (mr - ip) + 8;
         if (xnorm == 0) return -1.0;
@@ -380,62 +442,122 @@ struct Wave {
         alpha = alpha + beta;
         double tau = alpha / beta;
         double bip = rfl(b[ip]);
+        double u[3];
         part = 0.0;
-        for (int i = ip + 1 + L; i < mr; i += kWave) part = fma(b[i], Aat(i, jp) / alpha, part);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int i = L + k * kWave;
+            u[k] = c[k] / alpha;
+            if (i > ip && i < mr) part = fma(bv[k], u[k], part);
+        }
         double sm = bip + wsum(part);
         sm *= -tau;
         double A1 = -beta, b1 = bip + sm;
         flops += 3ull * (mr - ip);
         if (!(b1 / A1 > 0)) return -1.0;
-        __syncthreads();
+        ldbl *ci = A + ip * lda;
+        ldbl *cj = A + jp * lda;
         if (ip < mr - 1) {
-            if (ip != jp) {
-                for (int i = L; i < mr; i += kWave) {
-                    if (i < ip) {
-                        double t = Aat(i, ip); Aat(i, ip) = Aat(i, jp); Aat(i, jp) = t;
-                    } else if (i == ip) {
-                        b[ip] = b1;
-                        double t = Aat(ip, ip); Aat(ip, ip) = A1; Aat(ip, jp) = t;
-                    } else {
-                        double Aij = Aat(i, jp) / alpha, t = Aat(i, ip);
-                        Aat(i, ip) = Aij; Aat(i, jp) = t;
-                        b[i] = fma(sm, Aij, b[i]);
-                    }
-                }
-            } else {
-                for (int i = L; i < mr; i += kWave) {
-                    if (i == ip) { b[ip] = b1; Aat(ip, ip) = A1; }
-                    else if (i > ip) { double Aii = Aat(i, ip) / alpha; Aat(i, ip) = Aii; b[i] = fma(sm, Aii, b[i]); }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int i = L + k * kWave;
+                if (i >= mr) continue;
+                if (i < ip) {
+                    if (ip != jp) { const double t = ci[i]; ci[i] = c[k]; cj[i] = t; }
+                } else if (i == ip) {
+                    b[ip] = b1;
+                    if (ip != jp) { const double t = ci[ip]; ci[ip] = A1; cj[ip] = t; }
+                    else ci[ip] = A1;
+                } else {
+                    if (ip != jp) { const double t = ci[i]; ci[i] = u[k]; cj[i] = t; }
+                    else ci[i] = u[k];
+                    b[i] = fma(sm, u[k], bv[k]);
                 }
             }
             flops += 3ull * (mr - ip);
         } else {
             tau = 0.0;
             if (ip != jp)
-                for (int i = L; i < mr; i += kWave) { double t = Aat(i, ip); Aat(i, ip) = Aat(i, jp); Aat(i, jp) = t; }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int i = L + k * kWave;
+                    if (i < mr) { const double t = ci[i]; ci[i] = c[k]; cj[i] = t; }
+                }
         }
         __syncthreads();
         return tau;
     }
 
-    // apply_householder_dual! (:379-436): columns j > j1 (one per lane), rows [j1, m1).
+    // apply_householder_dual! (:379-436): columns j > j1 (one per lane), rows [j1, m1). The lane streams its
+    // own column and the reflector (column j1) with 16-byte loads over even-aligned row pairs; an odd first or
+    // last row is taken singly. Dot products use independent partial sums (the reference's @simd loops
+    // reassociate the same sums).
     __device__ __forceinline__ void apply_householder_dual(double tau, int j1, int m1) {
         if (j1 >= m1 - 1) return;
         const int j = L;
         if (j > j1 && j < n) {
-            double sm = Aat(j1, j);
-#pragma unroll 8
-            for (int i = j1 + 1; i < m1; ++i) sm = fma(Aat(i, j), Aat(i, j1), sm);
-            sm *= -tau;
-            Aat(j1, j) = Aat(j1, j) + sm;
-            double wj = 0.0;
-#pragma unroll 8
-            for (int i = j1 + 1; i < m1; ++i) {
-                double Aij = fma(sm, Aat(i, j1), Aat(i, j));
-                wj = fma(Aij, b[i], wj);
-                Aat(i, j) = Aij;
+            ldbl *cj = A + j * lda;
+            const ldbl *cu = A + j1 * lda;
+            cldv2 *vu = (cldv2 *)cu, *vb = (cldv2 *)b;
+            ldv2 *vj = (ldv2 *)cj;
+            const int lo = j1 + 1, hi = m1;
+            const int h = lo & 1, bs = lo + h, t = (hi - bs) & 1, be = hi - t;
+            const int q0 = bs >> 1, q1 = be >> 1;
+            double s0 = cj[j1], s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            if (h) s1 = cj[lo] * cu[lo];
+            if (t) s2 = cj[hi - 1] * cu[hi - 1];
+            int q = q0;
+            for (; q + 2 <= q1; q += 2) {
+                const dv2 a0 = vj[q], u0 = vu[q], a1 = vj[q + 1], u1 = vu[q + 1];
+                s0 = fma(a0.x, u0.x, s0);
+                s1 = fma(a0.y, u0.y, s1);
+                s2 = fma(a1.x, u1.x, s2);
+                s3 = fma(a1.y, u1.y, s3);
             }
-            w[j] = wj;
+            if (q < q1) {
+                const dv2 a0 = vj[q], u0 = vu[q];
+                s0 = fma(a0.x, u0.x, s0);
+                s1 = fma(a0.y, u0.y, s1);
+            }
+            const double sm = ((s0 + s1) + (s2 + s3)) * -tau;
+            cj[j1] = cj[j1] + sm;
+            double w0 = 0.0, w1 = 0.0;
+            if (h) {
+                const double a = fma(sm, cu[lo], cj[lo]);
+                w0 = a * b[lo];
+                cj[lo] = a;
+            }
+            q = q0;
+            // four row pairs per step: all loads are issued before the stores of the step
+            for (; q + 4 <= q1; q += 4) {
+                dv2 a[4], u[4], bb[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) { a[k] = vj[q + k]; u[k] = vu[q + k]; bb[k] = vb[q + k]; }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    a[k].x = fma(sm, u[k].x, a[k].x);
+                    a[k].y = fma(sm, u[k].y, a[k].y);
+                    w0 = fma(a[k].x, bb[k].x, w0);
+                    w1 = fma(a[k].y, bb[k].y, w1);
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) vj[q + k] = a[k];
+            }
+            for (; q < q1; ++q) {
+                dv2 a = vj[q];
+                const dv2 u = vu[q], bb = vb[q];
+                a.x = fma(sm, u.x, a.x);
+                a.y = fma(sm, u.y, a.y);
+                w0 = fma(a.x, bb.x, w0);
+                w1 = fma(a.y, bb.y, w1);
+                vj[q] = a;
+            }
+            if (t) {
+                const double a = fma(sm, cu[hi - 1], cj[hi - 1]);
+                w1 = fma(a, b[hi - 1], w1);
+                cj[hi - 1] = a;
+            }
+            w[j] = w0 + w1;
         }
         flops += 6ull * (m1 - j1) * (n - j1 - 1);
         __syncthreads();
@@ -444,42 +566,72 @@ struct Wave {
     // compute_dual! (:441-470): w_j = sum_{i >= j1} A_ij b_i for j >= j1, rows [j1, m1).
     __device__ __forceinline__ void compute_dual(int j1, int m1) {
         const int j = L;
-        if (j >= j1 && j < n) {
-            double sm = 0.0;
-#pragma unroll 8
-            for (int i = j1; i < m1; ++i) sm = fma(Aat(i, j), b[i], sm);
-            w[j] = sm;
+        if (j >= j1 && j < n && j1 < m1) {
+            const ldbl *cj = A + j * lda;
+            cldv2 *vj = (cldv2 *)cj, *vb = (cldv2 *)b;
+            const int lo = j1, hi = m1;
+            const int h = lo & 1, bs = lo + h, t = (hi - bs) & 1, be = hi - t;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            if (h) s0 = cj[lo] * b[lo];
+            if (t) s1 = cj[hi - 1] * b[hi - 1];
+            int q = bs >> 1;
+            const int q1 = be >> 1;
+            for (; q + 2 <= q1; q += 2) {
+                const dv2 a0 = vj[q], u0 = vb[q], a1 = vj[q + 1], u1 = vb[q + 1];
+                s0 = fma(a0.x, u0.x, s0);
+                s1 = fma(a0.y, u0.y, s1);
+                s2 = fma(a1.x, u1.x, s2);
+                s3 = fma(a1.y, u1.y, s3);
+            }
+            if (q < q1) {
+                const dv2 a0 = vj[q], u0 = vb[q];
+                s0 = fma(a0.x, u0.x, s0);
+                s1 = fma(a0.y, u0.y, s1);
+            }
+            w[j] = (s0 + s1) + (s2 + s3);
+        } else if (j >= j1 && j < n) {
+            w[j] = 0.0;
         }
         flops += 2ull * (m1 - j1) * (n - j1);
         __syncthreads();
     }
 
-    // solve_triangular_system! (:506-524) on R = A[0:k, 0:k]; lane i holds z_i in a register.
+    // solve_triangular_system! (:506-524) on R = A[0:k, 0:k]; lane i holds z_i in a register. The reference's
+    // zz[j] / A[j,j] is formed in lane j from its own reciprocal y = 1/A[j,j] as q = q0 + (z - q0 A[j,j]) y with
+    // q0 = z y: Markstein's correction, the correctly rounded quotient when y is the correctly rounded
+    // reciprocal. Every lane evaluates it on its own z (only lane j's is the final z_j at step j), so the
+    // sequential chain per column is fma, mul, fma, fma, readlane; column j - 2 is loaded during step j.
     __device__ __forceinline__ double tri_solve(int k, double zr) {
         if (k <= 0) return zr;
-        // loads for column j are issued one iteration ahead of their use
-        double ajj_n = Aat(k - 1, k - 1), alj_n = (L < k - 1) ? Aat(L, k - 1) : 0.0;
-        for (int j = k - 1; j >= 0; --j) {
-            const double ajj = rfl(ajj_n), alj = alj_n;
-            if (j > 0) {
-                ajj_n = Aat(j - 1, j - 1);
-                alj_n = (L < j - 1) ? Aat(L, j - 1) : 0.0;
-            }
-            double zj = readlane(zr, j);
-            double zi = -zj / ajj;
-            if (L < j) zr = fma(alj, zi, zr);
-            else if (L == j) zr = -zi;
+        const double aL = (L < k) ? Aat(L, L) : 1.0;
+        const double rL = 1.0 / aL;
+        double zf = zr;
+        // rows L < 64 of any column j < n stay inside the LDS allocation (b, zz, x, w follow the matrix)
+        // volatile: keeps each load at its place in the pipeline instead of being moved next to its use
+        volatile ldbl *Av = A;
+        double c0 = Av[(k - 1) * lda + L];
+        double c1 = Av[max(k - 2, 0) * lda + L];
+        for (int j = k - 1;;) {
+            const double c2 = Av[max(j - 2, 0) * lda + L];
+            const double q0 = zr * rL;
+            const double qL = fma(fma(-q0, aL, zr), rL, q0);
+            const double qj = readlane(qL, j);
+            zr = fma((L < j) ? c0 : 0.0, -qj, zr);
+            zf = (L == j) ? qj : zf;
+            if (j == 0) break;
+            --j;
+            c0 = c1;
+            c1 = c2;
         }
         flops += (unsigned long long)k * k;
-        return zr;
+        return (L < k) ? zf : zr;
     }
 
     // Remove coefficient imv from set P by Givens rotations and column shifts, then drop any further
-    // nonpositive coefficients (:729-779 / :965-1015).
-    __device__ __forceinline__ void nnls_delete(int &imv, int &nsetp, int mrows) {
+    // nonpositive coefficients (:729-779 / :965-1015). pidx / xp: the column and coefficient at position L.
+    __device__ __forceinline__ void nnls_delete(int &imv, int &nsetp, int mrows, int &pidx, double &xp) {
         for (;;) {
-            if (L == 0) x[idx[imv]] = 0.0;
-            __syncthreads();
+            if (L == imv) xp = 0.0;
             if (imv != nsetp - 1) {
                 for (int i = imv + 1; i <= nsetp - 1; ++i) {
                     double aa = rfl(Aat(i - 1, i)), bb = rfl(Aat(i, i));
@@ -507,15 +659,17 @@ struct Wave {
                     for (int j = imv; j < nsetp - 1; ++j) Aat(r, j) = Aat(r, j + 1);
                     Aat(r, nsetp - 1) = t;
                 }
-                if (L == 0) {
-                    int t = idx[imv];
-                    for (int j = imv; j < nsetp - 1; ++j) idx[j] = idx[j + 1];
-                    idx[nsetp - 1] = t;
-                }
+                // positions imv + 1 .. nsetp - 1 move down by one; position imv goes last
+                const int pim = readlane(pidx, imv);
+                const double xim = readlane(xp, imv);
+                const int pdn = __shfl_down(pidx, 1);
+                const double xdn = __shfl_down(xp, 1);
+                if (L >= imv && L < nsetp - 1) { pidx = pdn; xp = xdn; }
+                else if (L == nsetp - 1) { pidx = pim; xp = xim; }
                 __syncthreads();
             }
             nsetp -= 1;
-            bool bad = (L < nsetp) && (x[idx[L]] <= 0);
+            bool bad = (L < nsetp) && (xp <= 0);
             int f = first_set(ballot(bad));
             if (f < 0) break;
             imv = f;
@@ -524,32 +678,44 @@ struct Wave {
 
     // unsafe_nnls! (:605-825) and its Tikhonov twin unsafe_nnls!(work, λ) (:827-1061). Workspace rows
     // [0, M) with M = m (plain) or m + n (Tikhonov rows appended lazily when a column first enters).
+    // The permutation (position -> column), the coefficients of the active positions and the set of columns
+    // whose Tikhonov row is present live in registers (lane = position) and a wave-uniform bit mask; x, idx
+    // and diag are written back to LDS on exit.
     __device__ __forceinline__ void nnls_main(bool tikh, double lam, int &nsetp_out, double &rnorm_out) {
         const int M = tikh ? m + n : m;
         int mr = m;  // current number of active rows (the reference's m)
         int nsetp = 0, iter = 0;
         const int max_iter = 3 * n;
         bool terminated = false;
+        int pidx = L;           // column at position L (the reference's idx)
+        double xp = 0.0;        // its coefficient (zero outside the active set)
+        unsigned long long dmask = 0ull;  // columns whose Tikhonov row has been appended
         for (;;) {
             if (tikh ? (nsetp >= n) : (nsetp >= n || nsetp >= mr)) break;
-            int jmax = -1;
+            int jmax = -1, cmax = -1;
             double tau = 0.0;
             bool term = false;
+            double wv = (L >= nsetp && L < n) ? w[L] : 0.0;
             for (;;) {
-                double wv = (L >= nsetp && L < n) ? w[L] : 0.0;
-                if (!(wv > 0)) wv = 0.0;
-                int wi = (wv > 0) ? L : INT_MAX;
-                wargmax(wv, wi);
-                if (!(wv > 0)) { term = true; break; }
+                double v = (wv > 0) ? wv : 0.0;
+                int wi = (v > 0) ? L : INT_MAX;
+                wargmax(v, wi);
+                if (!(v > 0)) { term = true; break; }
                 jmax = wi;
-                bool newdiag = tikh && !diag[idx[jmax]];
+                cmax = readlane(pidx, jmax);
+                const bool newdiag = tikh && !((dmask >> cmax) & 1ull);
                 if (newdiag) {
                     if (L == 0) Aat(mr, jmax) = lam;
                     __syncthreads();
                 }
                 const int mrh = tikh ? min(mr + 1, M) : mr;
-                tau = construct_apply_householder(nsetp, jmax, mrh);
+                {
+                    DMI_PT0();
+                    tau = construct_apply_householder(nsetp, jmax, mrh);
+                    DMI_PT1(kPhConstruct);
+                }
                 if (tau >= 0) break;
+                if (L == jmax) wv = 0.0;
                 if (L == 0) {
                     w[jmax] = 0.0;
                     if (tikh && mr < M) Aat(mr, jmax) = 0.0;
@@ -557,21 +723,31 @@ struct Wave {
                 __syncthreads();
             }
             if (term) break;
-            if (tikh && !rfl(diag[idx[jmax]])) {
+            if (tikh && !((dmask >> cmax) & 1ull)) {
                 mr = min(mr + 1, M);
-                __syncthreads();
-                if (L == 0) diag[idx[jmax]] = 1;
+                dmask |= 1ull << cmax;
             }
             nsetp += 1;
-            __syncthreads();
-            if (L == 0) { int t = idx[nsetp - 1]; idx[nsetp - 1] = idx[jmax]; idx[jmax] = t; }
-            __syncthreads();
-            if (nsetp < n) apply_householder_dual(tau, nsetp - 1, mr);
+            {
+                // swap positions nsetp - 1 and jmax (both inactive, so both coefficients are zero)
+                const int pa = readlane(pidx, nsetp - 1);
+                if (L == nsetp - 1) pidx = cmax;
+                else if (L == jmax) pidx = pa;
+            }
+            {
+                DMI_PT0();
+                if (nsetp < n) apply_householder_dual(tau, nsetp - 1, mr);
+                DMI_PT1(kPhApply);
+            }
             for (int i = nsetp + L; i < mr; i += kWave) Aat(i, nsetp - 1) = 0.0;
             if (L == 0) w[nsetp - 1] = 0.0;
             __syncthreads();
             double zr = (L < nsetp) ? b[L] : 0.0;
-            zr = tri_solve(nsetp, zr);
+            {
+                DMI_PT0();
+                zr = tri_solve(nsetp, zr);
+                DMI_PT1(kPhTriSolve);
+            }
             bool dual_flag = false;
             for (;;) {
                 iter += 1;
@@ -579,8 +755,7 @@ struct Wave {
                 double t = INFINITY;
                 int ti = INT_MAX;
                 if (L < nsetp && zr <= 0) {
-                    double xi = x[idx[L]];
-                    double tt = -xi / (zr - xi);
+                    double tt = -xp / (zr - xp);
                     if (tt < 2.0) { t = tt; ti = L; }
                 }
                 wargmin(t, ti);
@@ -588,20 +763,33 @@ struct Wave {
                 dual_flag = true;
                 const double alpha = t;
                 int imv = ti;
-                if (L < nsetp) {
-                    int ix = idx[L];
-                    x[ix] = fma(alpha, zr - x[ix], x[ix]);
+                if (L < nsetp) xp = fma(alpha, zr - xp, xp);
+                {
+                    DMI_PT0();
+                    nnls_delete(imv, nsetp, mr, pidx, xp);
+                    DMI_PT1(kPhDelete);
                 }
-                __syncthreads();
-                nnls_delete(imv, nsetp, mr);
                 zr = (L < nsetp) ? b[L] : 0.0;
-                zr = tri_solve(nsetp, zr);
+                {
+                    DMI_PT0();
+                    zr = tri_solve(nsetp, zr);
+                    DMI_PT1(kPhTriSolve);
+                }
             }
             if (terminated) break;
-            if (dual_flag) compute_dual(nsetp, mr);
-            if (L < nsetp) x[idx[L]] = zr;
-            __syncthreads();
+            if (dual_flag) {
+                DMI_PT0();
+                compute_dual(nsetp, mr);
+                DMI_PT1(kPhDual);
+            }
+            if (L < nsetp) xp = zr;
         }
+        if (L < n) {
+            x[pidx] = (L < nsetp) ? xp : 0.0;
+            idx[L] = pidx;
+            diag[L] = (int)((dmask >> L) & 1ull);
+        }
+        __syncthreads();
         const int Mlim = tikh ? M : mr;
         double part = 0.0;
         for (int i = nsetp + L; i < Mlim; i += kWave) { double bi = b[i]; part = fma(bi, bi, part); }
@@ -612,6 +800,7 @@ struct Wave {
     // NNLSProblem solve! with the reference warm start (src/lsqnonneg.jl:30-84), or the Tikhonov form
     // (:86-164) when tikh. A0g: global row-major m x n basis; right-hand side b0 in LDS.
     __device__ __forceinline__ void solve(cgdbl *__restrict__ A0g, bool tikh, double mu, int &nsetp, double &rnorm) {
+        DMI_PT0();
         const int M = tikh ? m + n : m;
         double pd = 0.0;
         for (int i = L; i < m; i += kWave) { double a = A0g[i * n + n - 1]; pd += a * a; }
@@ -646,7 +835,12 @@ struct Wave {
         flops += 2ull * m * n + 6ull * m;
         solves += 1;
         __syncthreads();
-        nnls_main(tikh, mu, nsetp, rnorm);
+        DMI_PT1(kPhInit);
+        {
+            DMI_PT0();
+            nnls_main(tikh, mu, nsetp, rnorm);
+            DMI_PT1(kPhNnls);
+        }
     }
 
     // ------------------------- Tikhonov cache (src/lsqnonneg.jl:392-444) -------------------------
@@ -909,33 +1103,36 @@ __device__ __forceinline__ void reg_mdp(Wave &W, double delta, double &mu_out, d
 // eps * ||A||^2, far below the mu^2 >= exp(-16) scale at which gcv_dof uses them.
 __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
     const int m = W.m, n = W.n, L = W.L, lda = W.lda;
+    DMI_PT0();
     if (L < n)
         for (int i = 0; i < m; ++i) W.Aat(i, L) = W.A0[i * n + L];
     __syncthreads();
     N = min(m, n);
-    ldbl *G = W.A + (size_t)m * lda;  // N x N at rows [m, m + N)
+    // N x N Gram matrix at rows [m, m + N) of columns [0, N) (lda >= m + n leaves the room)
+    auto G = [&](int r, int c) -> ldbl & { return W.A[c * lda + m + r]; };
     if (L < N) {
         if (m >= n) {
             for (int c = 0; c < N; ++c) {
                 double s = 0.0;
                 for (int i = 0; i < m; ++i) s = fma(W.Aat(i, L), W.Aat(i, c), s);
-                G[L * lda + c] = s;
+                G(L, c) = s;
             }
         } else {
             for (int c = 0; c < N; ++c) {
                 double s = 0.0;
                 for (int j = 0; j < n; ++j) s = fma(W.Aat(L, j), W.Aat(c, j), s);
-                G[L * lda + c] = s;
+                G(L, c) = s;
             }
         }
     }
     W.flops += 2ull * N * N * (m >= n ? m : n);
     __syncthreads();
+    DMI_LAP(kPhGram);
     ldbl *dv = W.b, *ev = W.b + N, *vv = W.zz, *qv = W.zz + N;
     for (int k = 0; k < N - 2; ++k) {
-        double xr = (L > k && L < N) ? G[L * lda + k] : 0.0;
+        double xr = (L > k && L < N) ? G(L, k) : 0.0;
         double nrm = sqrt(wsum(xr * xr));
-        double x1 = rfl(G[(k + 1) * lda + k]);
+        double x1 = rfl(G(k + 1, k));
         if (nrm == 0.0) {
             if (L == 0) ev[k] = x1;
             __syncthreads();
@@ -956,19 +1153,20 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
         __syncthreads();
         double pr = 0.0;
         if (L > k && L < N)
-            for (int c = k + 1; c < N; ++c) pr = fma(G[L * lda + c], vv[c], pr);
+            for (int c = k + 1; c < N; ++c) pr = fma(G(L, c), vv[c], pr);
         double Kq = wsum(vr * pr);
         double qr = pr - Kq * vr;
         if (L < N) qv[L] = qr;
         __syncthreads();
         if (L > k && L < N)
-            for (int c = k + 1; c < N; ++c) G[L * lda + c] -= 2.0 * (vr * qv[c] + qr * vv[c]);
+            for (int c = k + 1; c < N; ++c) G(L, c) -= 2.0 * (vr * qv[c] + qr * vv[c]);
         W.flops += 8ull * (N - k) * (N - k);
         __syncthreads();
     }
-    if (L < N) dv[L] = G[L * lda + L];
-    if (N >= 2 && L == 0) ev[N - 2] = G[(N - 1) * lda + (N - 2)];
+    if (L < N) dv[L] = G(L, L);
+    if (N >= 2 && L == 0) ev[N - 2] = G(N - 1, N - 2);
     __syncthreads();
+    DMI_LAP(kPhTridiag);
     // Gershgorin interval
     double lo = INFINITY, hi = -INFINITY;
     if (L < N) {
@@ -981,36 +1179,56 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
     lo = wmind(lo);
     hi = wmaxd(hi);
     double gnorm = fmax(fabs(lo), fabs(hi));
+    // Each lane brackets eigenvalue t with count(a) <= t < count(b), count(x) = #eigenvalues < x, and splits the
+    // bracket in four per sweep: three Sturm sequences (LDL^T pivots q_i = (d_i - x) - e_{i-1}^2 / q_{i-1}) run
+    // interleaved, so their divides overlap. The divides must be exact: a pivot near zero amplifies any error in
+    // its quotient into later pivots' signs.
     double a = lo, b = hi;
     const int t = L;
     int it = 0;
-    for (; it < 200; ++it) {
-        double mid = 0.5 * (a + b);
-        bool done = !(t < N) || (b - a) <= fmax(2.0 * kEps * fmax(fabs(a), fabs(b)), kEps * gnorm) || mid <= a || mid >= b;
+    for (; it < 100; ++it) {
+        const double h = b - a;
+        const double x1 = a + 0.25 * h, x2 = a + 0.5 * h, x3 = a + 0.75 * h;
+        bool done = !(t < N) || h <= fmax(2.0 * kEps * fmax(fabs(a), fabs(b)), kEps * gnorm) || x2 <= a || x2 >= b;
         if (ballot(!done) == 0) break;
         if (!done) {
-            int cnt = 0;
-            double q = dv[0] - mid;
-            cnt += q < 0;
+            const double d0 = dv[0];
+            double q1 = d0 - x1, q2 = d0 - x2, q3 = d0 - x3;
+            int c1 = q1 < 0, c2 = q2 < 0, c3 = q3 < 0;
             for (int i = 1; i < N; ++i) {
-                if (q == 0.0) q = kEps * (fabs(ev[i - 1]) + kEps);
-                q = (dv[i] - mid) - (ev[i - 1] * ev[i - 1]) / q;
-                cnt += q < 0;
+                const double d = dv[i], e = ev[i - 1];
+                const double e2 = e * e, guard = kEps * (fabs(e) + kEps);
+                q1 = (q1 == 0.0) ? guard : q1;
+                q2 = (q2 == 0.0) ? guard : q2;
+                q3 = (q3 == 0.0) ? guard : q3;
+                q1 = (d - x1) - e2 / q1;
+                q2 = (d - x2) - e2 / q2;
+                q3 = (d - x3) - e2 / q3;
+                c1 += q1 < 0;
+                c2 += q2 < 0;
+                c3 += q3 < 0;
             }
-            if (cnt > t) b = mid;
-            else a = mid;
+            if (c1 > t) b = x1;
+            else if (c2 > t) { a = x1; b = x2; }
+            else if (c3 > t) { a = x2; b = x3; }
+            else a = x3;
         }
     }
-    W.flops += 5ull * N * N * it;
+    W.flops += 15ull * N * N * it;
     lam_lane = (t < N) ? fmax(0.5 * (a + b), 0.0) : 0.0;
     __syncthreads();
+    DMI_LAP(kPhSturm);
 }
 
 // lsqnonneg_gcv! with method :brent (:1136-1205)
 __device__ __forceinline__ void reg_gcv(Wave &W, double &mu_out, double &chi2_out) {
     double lam;
     int N;
-    gram_eigs(W, lam, N);
+    {
+        DMI_PT0();
+        gram_eigs(W, lam, N);
+        DMI_PT1(kPhGramEigs);
+    }
     const int m = W.m, n = W.n;
     const double low = (kEps * kEps) / m;
     W.cache_reset();
@@ -1440,13 +1658,14 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
     W.lda = p.lda;
     W.mw = p.mw;
     ldbl *s = (ldbl *)smem;
-    W.A = s; s += (size_t)p.mw * p.lda;
-    W.b = s; s += p.mw;
-    W.zz = s; s += p.mw;
+    // every double array has an even length, so each starts 16-byte aligned
+    W.A = s; s += (size_t)p.n * p.lda;
+    W.b = s; s += p.lda;
+    W.zz = s; s += p.lda;
     W.x = s; s += kMaxT2;
     W.w = s; s += kMaxT2;
     W.xf = s; s += kMaxT2;
-    W.b0 = s; s += p.m;
+    W.b0 = s; s += (p.m + 1) & ~1;
     W.cmu = s; s += kTikhCache;
     W.crn = s; s += kTikhCache;
     W.csemi = s; s += kTikhCache;
@@ -1463,6 +1682,7 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
     W.solves = 0;
     W.status = 0;
     W.ccur = 0;
+    DMI_PHASE_BEGIN();
     const int m = p.m, n = p.n, L = W.L;
     const int64_t V = p.V;
     for (;;) {
@@ -1496,14 +1716,19 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
                 fs.al = (L < p.K) ? p.alphas[L] : p.alphas[p.K - 1];
                 fs.u = NAN;
                 fs.du = NAN;
+                DMI_PT0();
                 alpha_v = fs.run();
+                DMI_PT1(kPhFlipSearch);
             }
+            DMI_PT0();
             voxel_basis(W, p.general != 0, alpha_v, p.beta, p.E1, (cgdbl *)p.E2, W.A0s, W.A0T);
+            DMI_PT1(kPhVoxelBasis);
             W.A0 = W.A0s;
             W.A0t = W.A0T;
         }
         // T2_distribution! (src/T2mapSEcorr.jl:475-505)
         double muv = 0.0, chi2v = 1.0;
+        DMI_PT0();
         if constexpr (REG == 0) W.plain_solve_to_xf();
         else if constexpr (REG == 1) reg_lcurve(W, muv, chi2v);
         else if constexpr (REG == 2) reg_gcv(W, muv, chi2v);
@@ -1513,6 +1738,7 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
             double delta = sqrt((double)m) * sigma;
             reg_mdp(W, delta, muv, chi2v);
         }
+        DMI_PT1(kPhRegTotal);
         // save_results! (:512-591)
         double xs = (L < n) ? W.xf[L] * max_signal : 0.0;
         if (L < n) W.xf[L] = xs;
@@ -1557,7 +1783,9 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
         if (p.decaybasis && p.mode != kModeFixed && L < n)
             for (int i = 0; i < m; ++i) p.decaybasis[I + V * (i + (int64_t)m * L)] = W.A0[i * n + L];
         __syncthreads();
+        DMI_PT1(kPhVoxel);
     }
+    DMI_PHASE_END();
     if (L == 0) {
         atomicAdd(&p.counters[0], W.flops);
         atomicAdd(&p.counters[1], W.solves);
@@ -1592,8 +1820,23 @@ __global__ void t2part_kernel(T2partParams p) {
 }  // namespace
 
 // ------------------------------------ host-side launchers ------------------------------------
+#ifdef DMI_PHASE_PROFILE
+extern "C" int dmi_phase_read(unsigned long long *out) {
+    return (int)hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), sizeof(g_phase));
+}
+extern "C" int dmi_phase_reset() {
+    unsigned long long z[2 * kPhCount] = {0};
+    return (int)hipMemcpyToSymbol(HIP_SYMBOL(g_phase), z, sizeof z);
+}
+#endif
+int t2map_lda(int m, int n) {
+    int ld = (m + n + 1) & ~1;
+    return (ld % 4 == 0) ? ld + 2 : ld;
+}
+
 size_t t2map_lds_bytes(int m, int n, int lda, int mw) {
-    size_t d = (size_t)mw * lda + 2 * (size_t)mw + 3 * kMaxT2 + (size_t)m + 3 * kTikhCache;
+    (void)mw;
+    size_t d = (size_t)n * lda + 2 * (size_t)lda + 3 * kMaxT2 + (size_t)((m + 1) & ~1) + 3 * kTikhCache;
     size_t i = 2 * kMaxT2 + 3 * (2 * kMaxAngles + 2);
     return d * sizeof(double) + i * sizeof(int);
 }

@@ -10,6 +10,8 @@ ROOT = PKG.parent            # decaes.jl_amd/
 REPO = ROOT.parent
 CSRC = ROOT / "csrc"
 LIB = PKG / "libdecaes_amd.so"
+# Same sources built with -DDMI_PHASE_PROFILE: per-phase shader-clock counters (decaes_amd.phase_profile)
+PHASE_LIB = PKG / "libdecaes_amd_phase.so"
 
 SOURCES = ["dmi_kernels.hip", "dmi_api.cpp", "dmi_synth.cpp"]
 
@@ -28,28 +30,34 @@ def hipcc() -> str:
     return str(Path(rocm) / "bin" / "hipcc")
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile every source into decaes_amd/libdecaes_amd.so (always recompiles)."""
+def build(verbose: bool = False, phase_profile: bool = False) -> Path:
+    """Compile every source into decaes_amd/libdecaes_amd.so (always recompiles). With phase_profile, build the
+    instrumented variant decaes_amd/libdecaes_amd_phase.so instead."""
     objs = []
-    obj_dir = ROOT / "_build"
+    obj_dir = ROOT / ("_build_phase" if phase_profile else "_build")
     obj_dir.mkdir(exist_ok=True)
+    out = PHASE_LIB if phase_profile else LIB
+    extra = ["-DDMI_PHASE_PROFILE"] if phase_profile else []
     for src in SOURCES:
         obj = obj_dir / (src + ".o")
-        cmd = [hipcc(), *HIPCC_FLAGS, "-c", "-I", str(REPO / "include"), "-I", str(CSRC), str(CSRC / src), "-o", str(obj)]
+        cmd = [hipcc(), *HIPCC_FLAGS, *extra, "-c", "-I", str(REPO / "include"), "-I", str(CSRC), str(CSRC / src),
+               "-o", str(obj)]
         if src.endswith(".cpp"):
             cmd = [c for c in cmd if c not in ("--offload-arch=gfx950",)]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
         objs.append(str(obj))
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = out.with_suffix(".so.tmp")
     link = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp), "-lpthread"]
     if verbose:
         print(" ".join(link), flush=True)
     subprocess.run(link, check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    import sys
+
+    print(build(verbose=True, phase_profile="--phase" in sys.argv[1:]))

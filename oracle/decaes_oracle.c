@@ -1096,7 +1096,6 @@ void orc_svdvals(const double *A, int m, int n, double *s) {
                 }
             }
         if (off <= EPS64) break;
-        if (getenv("ORC_SVD_DEBUG")) fprintf(stderr, "sweep %d off %g\n", sweep, off);
     }
     for (int j = 0; j < c; ++j) {
         double ss = 0;

@@ -8,6 +8,22 @@
  * Every function cites the reference file:line it restates (paths relative to the DECAES.jl v0.6.1 tree).
  * Julia is 1-based and column-major; this restatement keeps column-major storage and converts indices.
  * It is compiled with -ffp-contract=off and calls fma() exactly where the reference uses muladd/@muladd.
+ *
+ * What pins each part (tests/test_oracle_*.py; fixtures in tests/golden/ written by numpy/scipy only):
+ *   orc_epg_*            closed form at alpha = beta = 180; full-state complex EPG simulation; cross-form agreement
+ *   orc_epg_dalpha       central finite differences
+ *   orc_nnls, _tikh      scipy.optimize.nnls vectors; brute force over all active sets; KKT; Tikhonov gradient
+ *   orc_brent_*, bracket sin / sin^2 roots and minima (the reference's test/optimization.jl cases)
+ *   orc_hermite_minimize exact minimum of cubics
+ *   orc_lcurve_corner    mock (mu, 1/mu) curve: log mu* = 0 within 1e-3 (the reference's test/nnls.jl case)
+ *   orc_svdvals          numpy SVD of EPG bases; orc_gcv_dof: trace formula
+ *   orc_lsqnonneg_*      chi2 / mdp targets met; gcv optimum is a local minimum; lcurve solution is the
+ *                        Tikhonov solution at its mu; none is plain NNLS
+ *   orc_t2part, erfinv   closed forms on hand-built distributions; scipy.special.erfinv
+ *   orc_t2map            noiseless on-grid voxels recover alpha; threshold and NaN semantics
+ * Parity unpinned against the reference itself (no reference-held vectors; Julia cannot run here): the
+ * bisection trajectory of orc_flip_search, mu beyond solver tolerance in orc_lsqnonneg_{chi2,mdp,gcv}, the
+ * L-curve corner beyond the mock case, and hence the end-to-end maps of orc_t2map.
  */
 #ifndef DECAES_ORACLE_H
 #define DECAES_ORACLE_H
