@@ -61,19 +61,32 @@ __device__ __forceinline__ double readlane(double v, int l) {
 }
 __device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
-// Reductions: DPP within each 16-lane row (quad swaps, half-row mirror, row mirror), then the four row
-// results combined in a fixed order from readlane. Every lane of a row holds the same partial (each combine
-// is symmetric in its two operands), so the result is deterministic and uniform.
-template <int CTRL> __device__ __forceinline__ int dpp_i(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false); }
+// Reductions: DPP butterflies within each 16-lane row (quad swaps, half-row mirror, row mirror) leave every lane
+// of row r holding the row's result s_r; two row broadcasts then combine the rows as (s3 . s2) . (s1 . s0) into
+// lane 63, which is read out uniformly. Every combine is commutative, so this equals (s0 . s1) . (s2 . s3) bit for
+// bit, and the result is deterministic.
+// Full-mask permutations write every lane (a source lane that is switched off reads as 0), so no `old` operand
+// is needed.
+template <int CTRL> __device__ __forceinline__ int dpp_i(int v) { return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true); }
 template <int CTRL> __device__ __forceinline__ double dpp_d(double v) {
     unsigned long long u = __double_as_longlong(v);
     int lo = dpp_i<CTRL>((int)(u & 0xffffffffull)), hi = dpp_i<CTRL>((int)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// Row broadcasts write only the rows in ROWS; the other rows hold unspecified values, which wave_reduce never
+// lets reach lane 63 (row 3 is enabled in both broadcasts and its sources, lanes 47 and 31, are always valid).
+template <int CTRL, int ROWS> __device__ __forceinline__ double dpp_rows(double v) {
+    unsigned long long u = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_mov_dpp((int)(u & 0xffffffffull), CTRL, ROWS, 0xf, false);
+    int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, ROWS, 0xf, false);
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 constexpr int kQuadSwap1 = 0xB1;  // quad_perm [1,0,3,2]
 constexpr int kQuadSwap2 = 0x4E;  // quad_perm [2,3,0,1]
 constexpr int kHalfMirror = 0x141;
 constexpr int kRowMirror = 0x140;
+constexpr int kRowBcast15 = 0x142;  // lane 15 of each row -> the next row (rows 1 and 3 enabled)
+constexpr int kRowBcast31 = 0x143;  // lane 31 -> rows 2 and 3
 template <class Op> __device__ __forceinline__ double row_reduce(double v, Op op) {
     v = op(v, dpp_d<kQuadSwap1>(v));
     v = op(v, dpp_d<kQuadSwap2>(v));
@@ -81,46 +94,34 @@ template <class Op> __device__ __forceinline__ double row_reduce(double v, Op op
     v = op(v, dpp_d<kRowMirror>(v));
     return v;
 }
+template <class Op> __device__ __forceinline__ double wave_reduce(double v, Op op) {
+    v = row_reduce(v, op);
+    v = op(v, dpp_rows<kRowBcast15, 0xA>(v));
+    v = op(v, dpp_rows<kRowBcast31, 0xC>(v));
+    return readlane(v, 63);
+}
 __device__ __forceinline__ double wsum(double v) {
-    v = row_reduce(v, [](double a, double b) { return a + b; });
-    return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
+    return wave_reduce(v, [](double a, double b) { return a + b; });
 }
 __device__ __forceinline__ double wmaxd(double v) {
-    v = row_reduce(v, [](double a, double b) { return fmax(a, b); });
-    return fmax(fmax(readlane(v, 0), readlane(v, 16)), fmax(readlane(v, 32), readlane(v, 48)));
+    return wave_reduce(v, [](double a, double b) { return fmax(a, b); });
 }
 __device__ __forceinline__ double wmind(double v) {
-    v = row_reduce(v, [](double a, double b) { return fmin(a, b); });
-    return fmin(fmin(readlane(v, 0), readlane(v, 16)), fmin(readlane(v, 32), readlane(v, 48)));
+    return wave_reduce(v, [](double a, double b) { return fmin(a, b); });
 }
-// (value, index) pairs; Better(a, ia, b, ib) is a strict total order on pairs.
-template <int CTRL, class Better> __device__ __forceinline__ void pair_step(double &v, int &i, Better better) {
-    double ov = dpp_d<CTRL>(v);
-    int oi = dpp_i<CTRL>(i);
-    if (better(ov, oi, v, i)) { v = ov; i = oi; }
-}
-template <class Better> __device__ __forceinline__ void wpair(double &v, int &i, Better better) {
-    pair_step<kQuadSwap1>(v, i, better);
-    pair_step<kQuadSwap2>(v, i, better);
-    pair_step<kHalfMirror>(v, i, better);
-    pair_step<kRowMirror>(v, i, better);
-    double bv = readlane(v, 0);
-    int bi = readlane(i, 0);
-#pragma unroll
-    for (int r = 16; r < 64; r += 16) {
-        double rv = readlane(v, r);
-        int ri = readlane(i, r);
-        if (better(rv, ri, bv, bi)) { bv = rv; bi = ri; }
-    }
-    v = bv;
-    i = bi;
-}
-// (value desc, index asc) arg-max and (value asc, index asc) arg-min.
+// Arg-max / arg-min over lanes whose index i is the lane id (INT_MAX marks lanes that do not take part): the
+// extreme value, then the lowest lane holding it, i.e. (value desc | asc, index asc). Values are never NaN.
 __device__ __forceinline__ void wargmax(double &v, int &i) {
-    wpair(v, i, [](double a, int ia, double b, int ib) { return a > b || (a == b && ia < ib); });
+    const double vm = wmaxd(v);
+    const unsigned long long hit = __ballot(v == vm && i != INT_MAX);
+    v = vm;
+    i = hit ? __ffsll((long long)hit) - 1 : INT_MAX;
 }
 __device__ __forceinline__ void wargmin(double &v, int &i) {
-    wpair(v, i, [](double a, int ia, double b, int ib) { return a < b || (a == b && ia < ib); });
+    const double vm = wmind(v);
+    const unsigned long long hit = __ballot(v == vm && i != INT_MAX);
+    v = vm;
+    i = hit ? __ffsll((long long)hit) - 1 : INT_MAX;
 }
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int first_set(unsigned long long m) { return m ? __ffsll((long long)m) - 1 : -1; }
@@ -349,8 +350,9 @@ __global__ void transpose_kernel(const double *src, double *dst, int m, int n, i
 enum Phase : int {
     kPhConstruct = 1, kPhApply = 2, kPhTriSolve = 3, kPhDelete = 4, kPhDual = 5, kPhInit = 6, kPhNnls = 7,
     kPhGram = 8, kPhTridiag = 9, kPhFlipSearch = 10, kPhVoxelBasis = 11, kPhSturm = 12, kPhGramEigs = 13,
-    kPhRegTotal = 14, kPhVoxel = 15,
-    kPhCount = 16
+    kPhRegTotal = 14, kPhVoxel = 15, kPhSelect = 16, kPhFeasible = 17, kPhPostApply = 18,
+    kPhSwapApply = 19, kPhTriSolveOuter = 20,
+    kPhCount = 24
 };
 #ifdef DMI_PHASE_PROFILE
 __device__ unsigned long long g_phase[2 * kPhCount];
@@ -454,7 +456,9 @@ struct Wave {
         sm *= -tau;
         double A1 = -beta, b1 = bip + sm;
         flops += 3ull * (mr - ip);
-        if (!(b1 / A1 > 0)) return -1.0;
+        // b1 / A1 > 0 (A1 = -beta is nonzero): a sign test, with the divide kept for the quotient-underflow case
+        const bool pos = (b1 > 0 && A1 > 0) || (b1 < 0 && A1 < 0);
+        if (!(fabs(b1) >= 1e-290 ? pos : (b1 / A1 > 0))) return -1.0;
         ldbl *ci = A + ip * lda;
         ldbl *cj = A + jp * lda;
         if (ip < mr - 1) {
@@ -695,6 +699,7 @@ struct Wave {
             int jmax = -1, cmax = -1;
             double tau = 0.0;
             bool term = false;
+            DMI_PT0();
             double wv = (L >= nsetp && L < n) ? w[L] : 0.0;
             for (;;) {
                 double v = (wv > 0) ? wv : 0.0;
@@ -722,6 +727,7 @@ struct Wave {
                 }
                 __syncthreads();
             }
+            DMI_LAP(kPhSelect);
             if (term) break;
             if (tikh && !((dmask >> cmax) & 1ull)) {
                 mr = min(mr + 1, M);
@@ -739,16 +745,19 @@ struct Wave {
                 if (nsetp < n) apply_householder_dual(tau, nsetp - 1, mr);
                 DMI_PT1(kPhApply);
             }
+            DMI_LAP(kPhSwapApply);
             for (int i = nsetp + L; i < mr; i += kWave) Aat(i, nsetp - 1) = 0.0;
             if (L == 0) w[nsetp - 1] = 0.0;
             __syncthreads();
             double zr = (L < nsetp) ? b[L] : 0.0;
+            DMI_LAP(kPhPostApply);
             {
                 DMI_PT0();
                 zr = tri_solve(nsetp, zr);
                 DMI_PT1(kPhTriSolve);
             }
             bool dual_flag = false;
+            DMI_LAP(kPhTriSolveOuter);
             for (;;) {
                 iter += 1;
                 if (iter > max_iter) { status |= kStatusNnlsMaxIter; terminated = true; break; }
@@ -776,6 +785,7 @@ struct Wave {
                     DMI_PT1(kPhTriSolve);
                 }
             }
+            DMI_LAP(kPhFeasible);
             if (terminated) break;
             if (dual_flag) {
                 DMI_PT0();

@@ -23,10 +23,11 @@ WORKLOADS = {
     "chi2": ((64, 64, 8), 32, 10e-3, 40, (10e-3, 2.0), "chi2", 32, 3),
 }
 PHASES = [(1, "construct"), (2, "apply"), (3, "trisolve"), (4, "delete"), (5, "dual"), (6, "init"), (7, "nnls"),
+          (16, " select+constr"), (18, " post-apply"), (17, " feasibility"),
           (10, "flip search"), (11, "voxel basis"), (13, "gram eigs"), (8, " gram matrix"), (9, " tridiagonal"),
           (12, " sturm"), (14, "regularization"),
           (15, "reg + save")]
-NPH = 16
+NPH = 24
 
 
 def run(names):
