@@ -6,6 +6,7 @@ value = processed (above-threshold) voxels per second. Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3] [--no-cpu-baseline]
     python bench.py --copy-probe BYTES    # plain device copy of known size, the HBM-counter calibration run
+    python bench.py --serve [--qps 2,8,32] [--requests 16] [--max-slices 8]   # served stream: p50/p99 latency
 """
 from __future__ import annotations
 
@@ -95,6 +96,77 @@ def copy_probe(nbytes: int):
     print(json.dumps({"copy_bytes": a.numel() * 4, "GBps_read_plus_write": 2 * a.numel() * 4 / dt / 1e9}), flush=True)
 
 
+def serve(args):
+    """Served stream through the request loop: Poisson arrivals at each QPS of requests of 1..max_slices slices of
+    240x240 (nTE in {32, 48, 56}, Reg in {chi2, gcv, lcurve}, T2part on each), one JSON line with p50/p99 latency
+    and voxels/s per QPS."""
+    import threading
+
+    import numpy as np
+
+    import decaes_amd as dmi
+    from decaes_amd.server import Server
+
+    rng = np.random.default_rng(args.seed)
+    pool = {}
+    for nte, te in ((32, 10e-3), (48, 10e-3), (56, 7.9e-3)):
+        img, _ = dmi.synth_mse((240, 240, args.max_slices), nte, te, ncomp=4 if nte == 56 else 3, mask_fraction=0.6,
+                               seed=args.seed + nte)
+        pool[nte] = (img, te)
+    levels = []
+    for qps in [float(q) for q in args.qps.split(",")]:
+        reqs = []
+        for i in range(args.requests):
+            nte = int(rng.choice([32, 48, 56]))
+            reg = str(rng.choice(["chi2", "gcv", "lcurve"]))
+            ns = int(rng.integers(1, args.max_slices + 1))
+            z0 = int(rng.integers(0, args.max_slices - ns + 1))
+            img, te = pool[nte]
+            sub = np.asfortranarray(img[:, :, z0:z0 + ns, :])
+            o = dmi.T2mapOptions.from_image(sub, TE=te, nT2=40 if nte < 56 else 60,
+                                            T2Range=(10e-3, 2.0) if nte < 56 else (8e-3, 2.0), Reg=reg,
+                                            nRefAngles=32 if nte < 56 else 64, Chi2Factor=1.02, Silent=True)
+            po = dmi.T2partOptions.from_t2map(o, SPWin=(o.T2Range[0], 40e-3), MPWin=(40e-3, o.T2Range[1]))
+            reqs.append((sub, o, po))
+        gaps = rng.exponential(1.0 / qps, size=len(reqs))
+        lat = []
+        with Server() as srv:
+            handles = []
+            t0 = time.perf_counter()
+            waiter_done = threading.Event()
+
+            def waiter():
+                k = 0
+                while k < len(reqs):
+                    if k < len(handles):
+                        _, _, _, info = srv.wait(handles[k])
+                        lat.append(info["latency_ms"])
+                        k += 1
+                    else:
+                        time.sleep(0.001)
+                waiter_done.set()
+
+            th = threading.Thread(target=waiter)
+            th.start()
+            t_next = t0
+            for (sub, o, po), g in zip(reqs, gaps):
+                t_next += g
+                time.sleep(max(0.0, t_next - time.perf_counter()))
+                handles.append(srv.submit(sub, o, po))
+            th.join()
+            wall = time.perf_counter() - t0
+            st = srv.stats()
+        levels.append({"qps": qps, "requests": len(reqs), "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                       "voxels": st["voxels"], "voxels_per_s_busy": st["voxels"] / (st["busy_ms"] * 1e-3),
+                       "voxels_per_s_wall": st["voxels"] / wall, "failed": st["failed"]})
+        log(f"[serve] qps {qps}: p50 {st['p50_ms']:.0f} ms p99 {st['p99_ms']:.0f} ms, {st['voxels']} voxels, "
+            f"{levels[-1]['voxels_per_s_busy']:.0f} voxels/s busy")
+    print(json.dumps({"metric": "served latency (T2map+T2part requests)", "unit": "ms", "levels": levels,
+                      "config": {"requests_per_level": args.requests, "slices": f"1..{args.max_slices} of 240x240",
+                                 "nTE": [32, 48, 56], "Reg": ["chi2", "gcv", "lcurve"], "seed": args.seed},
+                      "data": "synthetic"}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,9 +176,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--serve", action="store_true")
+    ap.add_argument("--qps", default="2,8,32")
+    ap.add_argument("--requests", type=int, default=16)
+    ap.add_argument("--max-slices", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=7)
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.serve:
+        if args.gpus != 1:
+            print("the served stream runs on one GPU", file=sys.stderr)
+            sys.exit(2)
+        return serve(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); voxels shard across GPUs with no exchange "
               "step, so a multi-GPU figure is N independent single-GPU runs.", file=sys.stderr)

@@ -13,7 +13,7 @@ LIB = PKG / "libdecaes_amd.so"
 # Same sources built with -DDMI_PHASE_PROFILE: per-phase shader-clock counters (decaes_amd.phase_profile)
 PHASE_LIB = PKG / "libdecaes_amd_phase.so"
 
-SOURCES = ["dmi_kernels.hip", "dmi_api.cpp", "dmi_synth.cpp"]
+SOURCES = ["dmi_kernels.hip", "dmi_host.cpp", "dmi_api.cpp", "dmi_server.cpp", "dmi_synth.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",

@@ -29,6 +29,8 @@ _dp = C.POINTER(C.c_double)
 EXPORTS = (
     "dmi_create", "dmi_destroy", "dmi_last_error", "dmi_t2map_f64", "dmi_t2map_f64_device", "dmi_epg_basis_set",
     "dmi_t2part_f64", "dmi_t2part_f64_device", "dmi_last_stats", "dmi_synth_mse",
+    "dmi_server_create", "dmi_server_destroy", "dmi_server_submit", "dmi_server_wait", "dmi_server_poll",
+    "dmi_server_get_stats", "dmi_server_last_error", "dmi_server_request_error",
 )
 
 
@@ -60,6 +62,20 @@ class StatsC(C.Structure):
                 ("fp32_flops", C.c_int64), ("kernel_ms", C.c_double), ("total_ms", C.c_double)]
 
 
+class ServerConfigC(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_streams", C.c_int32), ("chunk_voxels", C.c_int32)]
+
+
+class RequestInfoC(C.Structure):
+    _fields_ = [("status", C.c_int32), ("_pad", C.c_int32), ("voxels", C.c_int64), ("queue_ms", C.c_double),
+                ("latency_ms", C.c_double)]
+
+
+class ServerStatsC(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64), ("voxels", C.c_int64),
+                ("busy_ms", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double)]
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -89,6 +105,17 @@ def load() -> C.CDLL:
         L.dmi_synth_mse.argtypes = [_dp, C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.c_double, C.c_double,
                                     C.c_int32, C.c_double, C.c_uint64, C.c_int32]
         L.dmi_synth_mse.restype = C.c_int64
+        L.dmi_server_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(ServerConfigC)]
+        L.dmi_server_destroy.argtypes = [vp]
+        L.dmi_server_submit.argtypes = [vp, _dp, C.POINTER(T2mapOptsC), _dp, C.POINTER(T2mapOutC),
+                                        C.POINTER(T2partOptsC), _dp, _dp, _dp, _dp, C.POINTER(C.c_int64)]
+        L.dmi_server_wait.argtypes = [vp, C.c_int64, C.POINTER(RequestInfoC)]
+        L.dmi_server_poll.argtypes = [vp, C.c_int64, C.POINTER(C.c_int32)]
+        L.dmi_server_get_stats.argtypes = [vp, C.POINTER(ServerStatsC)]
+        L.dmi_server_last_error.argtypes = [vp]
+        L.dmi_server_last_error.restype = C.c_char_p
+        L.dmi_server_request_error.argtypes = [vp, C.c_int64]
+        L.dmi_server_request_error.restype = C.c_char_p
         _lib = L
         return L
 

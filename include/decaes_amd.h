@@ -118,6 +118,54 @@ int dmi_t2part_f64_device(dmi_ctx *ctx, const double *d_dist, const dmi_t2part_o
 
 int dmi_last_stats(const dmi_ctx *ctx, dmi_stats *out);
 
+/* ---- Request loop ----
+ * A device-resident counterpart of the reference's warm server (api/decaes_server.jl:95-131, which keeps a Julia
+ * process alive and runs one CLI call at a time). Requests (a T2mapSEcorr call, optionally followed by
+ * T2partSEcorr on its distribution) are admitted while their device footprint fits the HBM budget, split into
+ * voxel chunks, and interleaved on the GPU: requests with the same grid options share a stream and its cached
+ * grid bases and take turns chunk by chunk; different option sets run on separate streams concurrently. A request
+ * that arrives while others run starts at the next chunk boundary. Each voxel's result is the same as in a
+ * dmi_t2map_f64 call (voxels are independent and the per-voxel computation does not depend on the batch).
+ * Host arrays passed to dmi_server_submit must stay valid until dmi_server_wait returns for that request. */
+typedef struct dmi_server dmi_server;
+
+typedef struct {
+    int64_t hbm_budget_bytes; /* device memory for admitted requests; <= 0 selects half of the free memory */
+    int32_t max_streams;      /* option sets in flight at once, one HIP stream each; <= 0 selects 4 */
+    int32_t chunk_voxels;     /* voxels per launch; <= 0 selects 65536 */
+} dmi_server_config;
+
+typedef struct {
+    int32_t status;           /* DMI_OK or the request's error code (dmi_server_request_error has the text) */
+    int32_t _pad;
+    int64_t voxels;           /* voxels above Threshold */
+    double queue_ms;          /* submit -> admitted */
+    double latency_ms;        /* submit -> outputs in host memory */
+} dmi_request_info;
+
+typedef struct {
+    int64_t submitted, completed, failed;
+    int64_t voxels;           /* voxels processed by completed requests */
+    double busy_ms;           /* wall time with at least one request admitted */
+    double p50_ms, p99_ms;    /* latency percentiles over completed requests */
+} dmi_server_stats;
+
+int dmi_server_create(dmi_server **out, int device, const dmi_server_config *cfg);
+/* Waits for every submitted request, then frees the server. */
+int dmi_server_destroy(dmi_server *s);
+/* Queues T2mapSEcorr (+ T2partSEcorr when popts is not NULL) on host arrays; returns at once with *id.
+ * Invalid options fail here with the reference's @assert message and nothing is queued. */
+int dmi_server_submit(dmi_server *s, const double *image, const dmi_t2map_opts *opts, const double *alpha_in,
+                      const dmi_t2map_out *out, const dmi_t2part_opts *popts, double *sfr, double *sgm, double *mfr,
+                      double *mgm, int64_t *id);
+/* Blocks until request id has finished (successfully or not). */
+int dmi_server_wait(dmi_server *s, int64_t id, dmi_request_info *info);
+/* *done = 1 once request id has finished. */
+int dmi_server_poll(dmi_server *s, int64_t id, int32_t *done);
+int dmi_server_get_stats(dmi_server *s, dmi_server_stats *out);
+const char *dmi_server_last_error(const dmi_server *s);
+const char *dmi_server_request_error(dmi_server *s, int64_t id);
+
 /* Seeded synthetic multi-echo spin-echo volume (host memory, no device work): per voxel a sum of
  * constant-flip EPG compartments plus Rician noise, zero outside an ellipsoid mask. Counter-based RNG keyed
  * by (seed, voxel), so any thread count gives the same bits. ncomp = 3 or 4 (the fourth is T2 ~ U(35,50) ms).
