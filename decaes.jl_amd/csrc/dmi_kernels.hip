@@ -1224,7 +1224,7 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
             else a = x3;
         }
     }
-    W.flops += 15ull * N * N * it;
+    W.flops += 10ull * N * N * it;  // two bisection steps per sweep, 5N^2 each
     lam_lane = (t < N) ? fmax(0.5 * (a + b), 0.0) : 0.0;
     __syncthreads();
     DMI_LAP(kPhSturm);
