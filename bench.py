@@ -173,6 +173,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
@@ -207,7 +208,8 @@ def main():
     img, nin = dmi.synth_mse(w["shape"], w["nTE"], w["TE"], ncomp=w["ncomp"], mask_fraction=w["mask"], seed=2024)
     log(f"[bench] synth done in {time.perf_counter() - t:.1f}s, {nin} voxels in mask")
     opts = dmi.T2mapOptions(MatrixSize=w["shape"], nTE=w["nTE"], TE=w["TE"], nT2=w["nT2"], T2Range=w["T2Range"],
-                            Reg=w["Reg"], nRefAngles=w["nRefAngles"], Chi2Factor=w["Chi2Factor"], Silent=True)
+                            Reg=w["Reg"], nRefAngles=w["nRefAngles"], Chi2Factor=w["Chi2Factor"], Silent=True,
+                            precision=args.precision)
     popts = dmi.T2partOptions.from_t2map(opts, SPWin=(w["T2Range"][0], 40e-3), MPWin=(40e-3, w["T2Range"][1]))
     dev = torch.device("cuda:0")
     # (X,Y,Z,nTE) column-major == contiguous (nTE,Z,Y,X)
@@ -253,14 +255,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32 EPG + f64 NNLS",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {X}x{Y}x{Z} nTE={w['nTE']} TE={w['TE']} nT2={w['nT2']} "
                                f"T2Range={w['T2Range']} Reg={w['Reg']} nRefAngles={w['nRefAngles']} "
-                               f"mask={w['mask']} precision=fp64",
+                               f"mask={w['mask']} precision={args.precision}",
                    "voxels_processed": nvox, "voxels_volume": X * Y * Z, "finite_gdn": finite,
                    "nnls_solves_per_voxel": stats[0]["nnls_solves"] / max(nvox, 1),
-                   "fp64_flops_per_voxel": flops / max(nvox, 1)},
+                   "fp64_flops_per_voxel": flops / max(nvox, 1),
+                   "fp32_flops_per_voxel": float(np.mean([s["fp32_flops"] for s in stats])) / max(nvox, 1)},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(args.workload),
                      "kernel": f"t2map_kernel<{dmi.api._lib.REG[w['Reg']]}>", "kernel_ms": kernel_ms,

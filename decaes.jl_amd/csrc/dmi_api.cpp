@@ -84,7 +84,7 @@ int run_t2map(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, const 
     rc = dmi_host::t2map_launch(c->err, p, c->vox.as<int64_t>(), nvox, c->lb, c->num_cu, c->stream);
     if (rc) return rc;
     HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
-    unsigned long long cnt[2] = {0, 0};
+    unsigned long long cnt[3] = {0, 0, 0};
     int status = 0;
     HIPCHECK(c, hipMemcpyAsync(cnt, c->lb.counters.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipMemcpyAsync(&status, c->lb.status.p, sizeof status, hipMemcpyDeviceToHost, c->stream));
@@ -95,6 +95,7 @@ int run_t2map(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, const 
     HIPCHECK(c, hipEventElapsedTime(&tms, c->ev[0], c->ev[3]));
     c->stats.nnls_solves = (int64_t)cnt[1];
     c->stats.fp64_flops = (int64_t)cnt[0];
+    c->stats.fp32_flops = (int64_t)cnt[2];
     c->stats.kernel_ms = kms;
     c->stats.total_ms = tms;
     if (status & dmi::kStatusLcurveOverflow)

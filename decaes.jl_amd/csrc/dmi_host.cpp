@@ -39,7 +39,7 @@ void DevBuf::release() {
 
 bool GridKey::operator==(const GridKey &o) const {
     auto same = [](double a, double b) { return a == b || (std::isnan(a) && std::isnan(b)); };
-    return m == o.m && n == o.n && K == o.K && general == o.general && same(TE, o.TE) && same(T1, o.T1) &&
+    return m == o.m && n == o.n && K == o.K && general == o.general && fp32 == o.fp32 && same(TE, o.TE) && same(T1, o.T1) &&
            same(beta, o.beta) && same(T2lo, o.T2lo) && same(T2hi, o.T2hi) && same(minref, o.minref) &&
            same(setflip, o.setflip);
 }
@@ -49,6 +49,7 @@ GridKey GridKey::of(const dmi_t2map_opts *o) {
     GridKey k;
     k.m = o->nTE; k.n = o->nT2; k.K = search ? o->nRefAngles : 1;
     k.general = o->RefConAngle != 180.0;
+    k.fp32 = o->precision == DMI_PREC_MIXED;
     k.TE = o->TE; k.T1 = o->T1; k.beta = o->RefConAngle; k.T2lo = o->T2Range[0]; k.T2hi = o->T2Range[1];
     k.minref = search ? o->MinRefAngle : NAN;
     k.setflip = search ? NAN : o->SetFlipAngle;
@@ -156,7 +157,7 @@ int Grids::ensure(std::string &err, const dmi_t2map_opts *o, hipStream_t s) {
     DMI_HIP(err, hipMemcpyAsync(logT2.p, lg.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
     DMI_HIP(err, hipMemcpyAsync(alphas.p, al.data(), sizeof(double) * K, hipMemcpyHostToDevice, s));
     dmi::GridParams g{};
-    g.m = m; g.n = n; g.K = K; g.general = k.general; g.beta = o->RefConAngle;
+    g.m = m; g.n = n; g.K = K; g.general = k.general; g.fp32 = k.fp32; g.beta = o->RefConAngle;
     g.E1 = std::exp(-(o->TE / 2) / o->T1);
     g.E2 = E2.as<double>();
     g.alphas = alphas.as<double>();
@@ -200,6 +201,7 @@ dmi::T2mapParams t2map_params(const dmi_t2map_opts *o, const Grids &g, const dou
     p.m = m; p.n = n; p.K = fixed ? 1 : o->nRefAngles; p.mineval = o->nRefAnglesMin; p.reg = o->Reg;
     p.mode = d_alpha ? dmi::kModeB1 : (fixed ? dmi::kModeFixed : dmi::kModeSearch);
     p.general = o->RefConAngle != 180.0;
+    p.fp32 = o->precision == DMI_PREC_MIXED;
     p.lda = dmi::t2map_lda(m, n);
     p.mw = m + n;
     p.save_resnorm = o->SaveResidualNorm; p.save_decaycurve = o->SaveDecayCurve;

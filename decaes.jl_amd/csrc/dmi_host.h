@@ -33,7 +33,7 @@ struct DevBuf {
 // Everything the grid bases depend on (the reference builds them per task in thread_buffer_maker,
 // src/T2mapSEcorr.jl:596-614).
 struct GridKey {
-    int m = -1, n = -1, K = -1, general = -1;
+    int m = -1, n = -1, K = -1, general = -1, fp32 = -1;
     double TE = 0, T1 = 0, beta = 0, T2lo = 0, T2hi = 0, minref = 0, setflip = 0;
     bool operator==(const GridKey &o) const;
     static GridKey of(const dmi_t2map_opts *o);

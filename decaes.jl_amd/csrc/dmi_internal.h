@@ -22,6 +22,7 @@ enum VoxelMode : int { kModeSearch = 0, kModeB1 = 1, kModeFixed = 2 };
 
 struct GridParams {
     int m, n, K;
+    int fp32;             // 1: FP32 recursion (mixed precision)
     int general;          // 1: RefConAngle != 180 (general refocusing train)
     double beta;          // RefConAngle
     double E1;            // exp(-(TE/2)/T1)
@@ -34,6 +35,7 @@ struct GridParams {
 
 struct T2mapParams {
     int m, n, K, mineval, reg, mode, general;
+    int fp32;             // 1: FP32 EPG recursion for the voxel basis (mixed precision)
     int lda, mw;          // LDS workspace: mw = m + n rows, column-major with column stride lda = t2map_lda(m, n)
     int save_resnorm, save_decaycurve, save_regparam, save_basis;
     int64_t V;            // voxels in the volume
@@ -54,7 +56,7 @@ struct T2mapParams {
     double *scratch;      // per-workgroup slots
     int64_t slot_stride;  // doubles per slot
     double *dist, *gdn, *ggm, *gva, *fnr, *snr, *alpha, *resnorm, *decaycurve, *mu, *chi2f, *decaybasis;
-    unsigned long long *counters; // [0] fp64 flops, [1] nnls solves
+    unsigned long long *counters; // [0] fp64 flops, [1] nnls solves, [2] fp32 flops
     int *status;
 };
 

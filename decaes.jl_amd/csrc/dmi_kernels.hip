@@ -202,6 +202,34 @@ __device__ __forceinline__ void tsincos(Dual x, Dual *s, Dual *c) {
     *c = {cv, -sv * x.d};
 }
 __device__ __forceinline__ void tsincos(double x, double *s, double *c) { sincos(x, s, c); }
+// FP32 twins for the mixed-precision path: the recursion runs in FP32 (coefficients are formed in FP64 and
+// rounded once), results are stored as FP64.
+struct DualF {
+    float v, d;
+};
+__device__ __forceinline__ DualF operator+(DualF a, DualF b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ DualF operator-(DualF a, DualF b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ DualF operator-(DualF a) { return {-a.v, -a.d}; }
+__device__ __forceinline__ DualF operator*(DualF a, DualF b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ DualF operator*(DualF a, double s) { return {a.v * (float)s, a.d * (float)s}; }
+__device__ __forceinline__ DualF operator/(DualF a, double s) { return {a.v / (float)s, a.d / (float)s}; }
+__device__ __forceinline__ DualF tfma(DualF a, DualF b, DualF c) { return {fmaf(a.v, b.v, c.v), a.d * b.v + a.v * b.d + c.d}; }
+__device__ __forceinline__ float tfma(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ DualF tabs(DualF a) { return signbit(a.v) ? -a : a; }
+__device__ __forceinline__ float tabs(float a) { return fabsf(a); }
+__device__ __forceinline__ void tsincos(DualF x, DualF *s, DualF *c) {
+    double sv, cv;
+    sincos((double)x.v, &sv, &cv);
+    *s = {(float)sv, (float)(cv * x.d)};
+    *c = {(float)cv, (float)(-sv * x.d)};
+}
+__device__ __forceinline__ void tsincos(float x, float *s, float *c) {
+    double sv, cv;
+    sincos((double)x, &sv, &cv);
+    *s = (float)sv;
+    *c = (float)cv;
+}
+
 template <typename T> struct Num;
 template <> struct Num<double> {
     static __device__ __forceinline__ double c(double v) { return v; }
@@ -218,6 +246,25 @@ template <> struct Num<Dual> {
     }
     static __device__ __forceinline__ double val(Dual x) { return x.v; }
 };
+template <> struct Num<float> {
+    static __device__ __forceinline__ float c(double v) { return (float)v; }
+    static __device__ __forceinline__ float var(double v, double) { return (float)v; }
+    static __device__ __forceinline__ void put(float x, gdbl *v, gdbl *, int64_t o) { v[o] = (double)x; }
+    static __device__ __forceinline__ double val(float x) { return (double)x; }
+};
+template <> struct Num<DualF> {
+    static __device__ __forceinline__ DualF c(double v) { return {(float)v, 0.0f}; }
+    static __device__ __forceinline__ DualF var(double v, double d) { return {(float)v, (float)d}; }
+    static __device__ __forceinline__ void put(DualF x, gdbl *v, gdbl *d, int64_t o) {
+        v[o] = (double)x.v;
+        if (d) d[o] = (double)x.d;
+    }
+    static __device__ __forceinline__ double val(DualF x) { return (double)x.v; }
+};
+template <int LPC> __device__ __forceinline__ float shup(float a) { return __shfl_up(a, 1, LPC); }
+template <int LPC> __device__ __forceinline__ float shdn(float a) { return __shfl_down(a, 1, LPC); }
+template <int LPC> __device__ __forceinline__ DualF shup(DualF a) { return {__shfl_up(a.v, 1, LPC), __shfl_up(a.d, 1, LPC)}; }
+template <int LPC> __device__ __forceinline__ DualF shdn(DualF a) { return {__shfl_down(a.v, 1, LPC), __shfl_down(a.d, 1, LPC)}; }
 template <int LPC> __device__ __forceinline__ double shup(double a) { return __shfl_up(a, 1, LPC); }
 template <int LPC> __device__ __forceinline__ double shdn(double a) { return __shfl_down(a, 1, LPC); }
 template <int LPC> __device__ __forceinline__ Dual shup(Dual a) { return {__shfl_up(a.v, 1, LPC), __shfl_up(a.d, 1, LPC)}; }
@@ -325,9 +372,15 @@ __global__ __launch_bounds__(64) void grid_basis_kernel(GridParams p) {
     const int jj = valid ? j : p.n - 1;
     const int64_t base = (int64_t)k * p.m * p.n + jj;
     gdbl *outT = p.AsT ? (gdbl *)p.AsT + (int64_t)k * p.m * p.n + (int64_t)jj * p.m : nullptr;
-    if (p.dAs)
+    if (p.dAs && p.fp32)
+        epg_curve<LPC, DualF>(p.general, p.alphas[k], p.beta, p.E1, p.E2[jj], p.m, s, valid, (gdbl *)p.As + base,
+                              (gdbl *)p.dAs + base, p.n, outT);
+    else if (p.dAs)
         epg_curve<LPC, Dual>(p.general, p.alphas[k], p.beta, p.E1, p.E2[jj], p.m, s, valid, (gdbl *)p.As + base,
                              (gdbl *)p.dAs + base, p.n, outT);
+    else if (p.fp32)
+        epg_curve<LPC, float>(p.general, p.alphas[k], p.beta, p.E1, p.E2[jj], p.m, s, valid, (gdbl *)p.As + base,
+                              nullptr, p.n, outT);
     else
         epg_curve<LPC, double>(p.general, p.alphas[k], p.beta, p.E1, p.E2[jj], p.m, s, valid, (gdbl *)p.As + base,
                                nullptr, p.n, outT);
@@ -411,7 +464,7 @@ struct Wave {
     cgdbl *A0;   // basis used by the regularization solves (A0s or the fixed basis), row-major
     cgdbl *A0t;  // the same basis with columns contiguous
     int ccur;
-    unsigned long long flops, solves;
+    unsigned long long flops, flops32, solves;
     int status;
 
     __device__ __forceinline__ ldbl &Aat(int i, int j) { return A[j * lda + i]; }
@@ -1636,8 +1689,9 @@ struct FlipSearch {
 };
 
 // Basis at one flip angle: n EPG curves, two per wave (32-lane groups) when the echo window fits.
-__device__ __forceinline__ void voxel_basis(Wave &W, bool general, double alpha, double beta, double E1, cgdbl *E2,
-                            gdbl *A0out, gdbl *A0T) {
+template <typename T>
+__device__ __forceinline__ void voxel_basis_t(Wave &W, bool general, double alpha, double beta, double E1, cgdbl *E2,
+                                              gdbl *A0out, gdbl *A0T) {
     const int m = W.m, n = W.n, L = W.L;
     if (m / 2 + 2 <= 32) {
         const int h = L >> 5, s = L & 31;
@@ -1645,15 +1699,21 @@ __device__ __forceinline__ void voxel_basis(Wave &W, bool general, double alpha,
             int j = jp + h;
             bool valid = j < n;
             int jj = valid ? j : n - 1;
-            epg_curve<32, double>(general, alpha, beta, E1, E2[jj], m, s, valid, A0out + jj, nullptr, n,
-                                  A0T + (int64_t)jj * m);
+            epg_curve<32, T>(general, alpha, beta, E1, E2[jj], m, s, valid, A0out + jj, nullptr, n, A0T + (int64_t)jj * m);
         }
     } else {
         for (int j = 0; j < n; ++j)
-            epg_curve<64, double>(general, alpha, beta, E1, E2[j], m, L, true, A0out + j, nullptr, n, A0T + (int64_t)j * m);
+            epg_curve<64, T>(general, alpha, beta, E1, E2[j], m, L, true, A0out + j, nullptr, n, A0T + (int64_t)j * m);
     }
+}
+__device__ __forceinline__ void voxel_basis(Wave &W, bool fp32, bool general, double alpha, double beta, double E1,
+                                            cgdbl *E2, gdbl *A0out, gdbl *A0T) {
+    if (fp32) voxel_basis_t<float>(W, general, alpha, beta, E1, E2, A0out, A0T);
+    else voxel_basis_t<double>(W, general, alpha, beta, E1, E2, A0out, A0T);
     // EPG op count of the reference's split recursion: ~13 flops per retained state per echo
-    W.flops += (unsigned long long)n * 13ull * ((unsigned long long)m * m / 4 + m);
+    const unsigned long long f = (unsigned long long)W.n * 13ull * ((unsigned long long)W.m * W.m / 4 + W.m);
+    if (fp32) W.flops32 += f;
+    else W.flops += f;
     __syncthreads();
 }
 
@@ -1689,6 +1749,7 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
     W.cx = W.A0T + (int64_t)p.m * p.n;
     W.lc = W.cx + (int64_t)kTikhCache * p.n;
     W.flops = 0;
+    W.flops32 = 0;
     W.solves = 0;
     W.status = 0;
     W.ccur = 0;
@@ -1731,7 +1792,7 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
                 DMI_PT1(kPhFlipSearch);
             }
             DMI_PT0();
-            voxel_basis(W, p.general != 0, alpha_v, p.beta, p.E1, (cgdbl *)p.E2, W.A0s, W.A0T);
+            voxel_basis(W, p.fp32 != 0, p.general != 0, alpha_v, p.beta, p.E1, (cgdbl *)p.E2, W.A0s, W.A0T);
             DMI_PT1(kPhVoxelBasis);
             W.A0 = W.A0s;
             W.A0t = W.A0T;
@@ -1799,6 +1860,7 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
     if (L == 0) {
         atomicAdd(&p.counters[0], W.flops);
         atomicAdd(&p.counters[1], W.solves);
+        atomicAdd(&p.counters[2], W.flops32);
         if (W.status) atomicOr(p.status, W.status);
     }
 }

@@ -174,3 +174,30 @@ def test_threshold_edge_cases(dmi):
     img2[0, 0, 0, 3] = np.nan  # a NaN echo propagates to that voxel's outputs, others unaffected
     maps2, dist2 = dmi.T2mapSEcorr(img2, opts)
     assert np.sum(~np.isnan(maps2["gdn"])) >= shape[0] * shape[1] - 1
+
+
+@pytest.mark.parametrize("name", ["c3_lcurve", "c2_gcv"])
+def test_mixed_precision_error(dmi, oracle, name):
+    """FP32 EPG (grid bases and the voxel basis) with FP64 NNLS and regularization, against the FP64 oracle:
+    alpha <= 0.1 deg and MWF <= 1e-3 absolute per voxel; every output's error distribution is written out."""
+    print(f"[parity] case=mixed_{name}", flush=True)
+    shape, m, TE, n, T2R, reg, K, extra, ncomp, frac = CASES[name]
+    img, opts, maps, dist, omaps, odist, nv = _run(dmi, oracle, name, precision="mixed")
+    mask, rel, da = _errors(maps, dist, omaps, odist, T2R)
+    pg = oracle.t2part(dist, T2R, (T2R[0], 40e-3), (40e-3, T2R[1]))
+    po = oracle.t2part(odist, T2R, (T2R[0], 40e-3), (40e-3, T2R[1]))
+    mwf = np.abs(pg["sfr"][mask] - po["sfr"][mask])
+    sgm = np.abs(pg["sgm"][mask] - po["sgm"][mask]) / np.abs(po["sgm"][mask])
+    chi = np.abs(maps["chi2factor"][mask] - omaps["chi2factor"][mask])
+
+    def q(x):
+        return dict(max=float(np.nanmax(x)), p99=float(np.nanquantile(x, 0.99)), median=float(np.nanmedian(x)))
+
+    s = dict(case=f"mixed_{name}", voxels=int(mask.sum()), alpha_abs_deg=q(da), dist_rel_l2=q(rel), mwf_abs=q(mwf),
+             sgm_rel=q(sgm), chi2factor_abs=q(chi))
+    OUT.mkdir(exist_ok=True)
+    with open(OUT / f"parity_mixed_{name}.json", "w") as f:
+        json.dump(s, f, indent=1)
+    print(json.dumps(s), flush=True)
+    assert np.max(da) <= 0.1, s
+    assert np.max(mwf) <= 1e-3, s
