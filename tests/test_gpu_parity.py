@@ -201,3 +201,31 @@ def test_mixed_precision_error(dmi, oracle, name):
     print(json.dumps(s), flush=True)
     assert np.max(da) <= 0.1, s
     assert np.max(mwf) <= 1e-3, s
+
+
+@pytest.mark.parametrize("shape,m,n,K,reg", [
+    ((3, 2, 1), 124, 64, 64, "gcv"),       # device maximums: 124 echoes, 64 T2 bins, 64 angles (3 rows per lane)
+    ((2, 2, 1), 124, 64, 64, "lcurve"),
+    ((4, 3, 1), 4, 2, 2, "chi2"),          # reference minimums: 4 echoes, 2 T2 bins, 2 angles
+    ((1, 1, 1), 32, 40, 32, "none"),       # a single voxel
+    ((5, 1, 3), 20, 64, 16, "mdp"),        # n > m
+])
+def test_size_limits(dmi, oracle, shape, m, n, K, reg):
+    print(f"[parity] case=limits {shape} m={m} n={n} K={K} {reg}", flush=True)
+    TE, T2R = 10e-3, (10e-3, 2.0)
+    img, _ = dmi.synth_mse(shape, m, TE, mask_fraction=1.0, seed=17)
+    extra = dict(Chi2Factor=1.02) if reg == "chi2" else dict(NoiseLevel=1e-3) if reg == "mdp" else {}
+    opts = dmi.T2mapOptions.from_image(img, TE=TE, nT2=n, T2Range=T2R, Reg=reg, nRefAngles=K,
+                                       nRefAnglesMin=min(5, K), Silent=True, SaveRegParam=True, **extra)
+    maps, dist = dmi.T2mapSEcorr(img, opts)
+    oo = oracle.make_t2map_opts(shape, m, TE, n, T2R, reg, nRefAngles=K, nRefAnglesMin=min(5, K),
+                                Chi2Factor=extra.get("Chi2Factor"), NoiseLevel=extra.get("NoiseLevel"))
+    omaps, odist, nv = oracle.t2map(img, oo, nthreads=os.cpu_count())
+    assert np.array_equal(np.isnan(dist), np.isnan(odist))
+    mask, rel, da = _errors(maps, dist, omaps, odist, T2R)
+    print(f"[parity] limits: {mask.sum()} voxels, max dist rel {rel.max():.2e}, max alpha {da.max():.2e}", flush=True)
+    assert np.max(da) <= 1e-2
+    if reg == "lcurve":
+        assert np.mean(rel <= 1e-6) >= 0.5
+    else:
+        assert np.mean(rel <= 1e-6) >= 0.9
