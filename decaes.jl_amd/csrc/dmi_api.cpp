@@ -30,8 +30,9 @@ struct dmi_ctx {
     dmi_host::Grids grids;
     dmi_host::LaunchBufs lb;
     DevBuf vox;
-    // host-array staging
+    // host-array staging (FP64 working copies, FP32 transfer buffers)
     DevBuf img, ain, out[12], t2p[5];
+    DevBuf f32;
     DevBuf weights, tlog;
 };
 
@@ -162,7 +163,7 @@ int dmi_destroy(dmi_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->grids.release();
     c->lb.release();
-    DevBuf *bufs[] = {&c->vox, &c->img, &c->ain, &c->weights, &c->tlog};
+    DevBuf *bufs[] = {&c->vox, &c->img, &c->ain, &c->weights, &c->tlog, &c->f32};
     for (auto *b : bufs) b->release();
     for (auto &b : c->out) b.release();
     for (auto &b : c->t2p) b.release();
@@ -231,6 +232,82 @@ int dmi_t2map_f64(dmi_ctx *c, const double *image, const dmi_t2map_opts *o, cons
         if (hp[i]) HIPCHECK(c, hipMemcpyAsync(hp[i], dp[i], sizeof(double) * cnt[i], hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return rc;
+}
+
+// Host Float32 array -> device FP64 (through the FP32 transfer buffer)
+static int widen_in(dmi_ctx *c, const float *h, DevBuf &d, size_t n) {
+    HIPCHECK(c, d.ensure(sizeof(double) * std::max<size_t>(n, 1)));
+    HIPCHECK(c, c->f32.ensure(sizeof(float) * std::max<size_t>(n, 1)));
+    HIPCHECK(c, hipMemcpyAsync(c->f32.p, h, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, dmi::launch_widen(c->f32.as<float>(), d.as<double>(), (int64_t)n, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));  // the transfer buffer is reused by the next array
+    return DMI_OK;
+}
+
+// Device FP64 -> host Float32
+static int narrow_out(dmi_ctx *c, const DevBuf &d, float *h, size_t n) {
+    HIPCHECK(c, c->f32.ensure(sizeof(float) * std::max<size_t>(n, 1)));
+    HIPCHECK(c, dmi::launch_narrow(d.as<double>(), c->f32.as<float>(), (int64_t)n, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(h, c->f32.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return DMI_OK;
+}
+
+int dmi_t2map_f32(dmi_ctx *c, const float *image, const dmi_t2map_opts *o, const float *alpha_in,
+                  const dmi_t2map_out_f32 *out) {
+    if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHECK(c, hipSetDevice(c->device));
+    int rc = dmi_host::validate_t2map(c->err, o);
+    if (rc) return rc;
+    if (!image || !out) return fail(c, DMI_EINVAL, "image and out are required");
+    const int m = o->nTE, n = o->nT2;
+    const int64_t V = o->MatrixSize[0] * o->MatrixSize[1] * o->MatrixSize[2];
+    const bool fixed = !std::isnan(o->SetFlipAngle);
+    if ((rc = widen_in(c, image, c->img, (size_t)V * m))) return rc;
+    if (alpha_in && (rc = widen_in(c, alpha_in, c->ain, (size_t)V))) return rc;
+    float *hp[12] = {out->dist, out->gdn, out->ggm, out->gva, out->fnr, out->snr, out->alpha,
+                     o->SaveResidualNorm ? out->resnorm : nullptr, o->SaveDecayCurve ? out->decaycurve : nullptr,
+                     o->SaveRegParam ? out->mu : nullptr, o->SaveRegParam ? out->chi2factor : nullptr,
+                     o->SaveNNLSBasis ? out->decaybasis : nullptr};
+    const size_t cnt[12] = {(size_t)V * n, (size_t)V, (size_t)V, (size_t)V, (size_t)V, (size_t)V, (size_t)V, (size_t)V,
+                            (size_t)V * m, (size_t)V, (size_t)V, fixed ? (size_t)m * n : (size_t)V * m * n};
+    double *dp[12] = {nullptr};
+    for (int i = 0; i < 12; ++i) {
+        if (!hp[i]) continue;
+        if ((rc = widen_in(c, hp[i], c->out[i], cnt[i]))) return rc;  // keeps the caller's values below threshold
+        dp[i] = c->out[i].as<double>();
+    }
+    dmi_t2map_out d{dp[0], dp[1], dp[2], dp[3], dp[4], dp[5], dp[6], dp[7], dp[8], dp[9], dp[10], dp[11]};
+    rc = run_t2map(c, c->img.as<double>(), o, alpha_in ? c->ain.as<double>() : nullptr, &d);
+    if (rc && rc != DMI_ECAPACITY) return rc;
+    for (int i = 0; i < 12; ++i) {
+        if (!hp[i]) continue;
+        int r2 = narrow_out(c, c->out[i], hp[i], cnt[i]);
+        if (r2) return r2;
+    }
+    return rc;
+}
+
+int dmi_t2part_f32(dmi_ctx *c, const float *dist, const dmi_t2part_opts *o, float *sfr, float *sgm, float *mfr,
+                   float *mgm) {
+    if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHECK(c, hipSetDevice(c->device));
+    if (!o) return fail(c, DMI_EINVAL, "opts is NULL");
+    if (!dist || !sfr || !sgm || !mfr || !mgm) return fail(c, DMI_EINVAL, "dist and all four outputs are required");
+    const int64_t V = o->MatrixSize[0] * o->MatrixSize[1] * o->MatrixSize[2];
+    int rc = widen_in(c, dist, c->t2p[0], (size_t)V * std::max(o->nT2, 1));
+    if (rc) return rc;
+    float *hp[4] = {sfr, sgm, mfr, mgm};
+    for (int i = 0; i < 4; ++i)
+        if ((rc = widen_in(c, hp[i], c->t2p[1 + i], (size_t)V))) return rc;
+    rc = run_t2part(c, c->t2p[0].as<double>(), o, c->t2p[1].as<double>(), c->t2p[2].as<double>(),
+                    c->t2p[3].as<double>(), c->t2p[4].as<double>());
+    if (rc) return rc;
+    for (int i = 0; i < 4; ++i)
+        if ((rc = narrow_out(c, c->t2p[1 + i], hp[i], (size_t)V))) return rc;
+    return DMI_OK;
 }
 
 int dmi_epg_basis_set(dmi_ctx *c, const dmi_t2map_opts *o, double *out) {

@@ -80,6 +80,9 @@ hipError_t launch_select(const double *image, int64_t V, double threshold, int64
                          hipStream_t s);
 hipError_t launch_t2map(const T2mapParams &p, int grid, size_t lds, hipStream_t s);
 hipError_t launch_t2part(const T2partParams &p, hipStream_t s);
+// Float32 <-> Float64 element-wise conversion (Float32 entry points)
+hipError_t launch_widen(const float *src, double *dst, int64_t n, hipStream_t s);
+hipError_t launch_narrow(const double *src, float *dst, int64_t n, hipStream_t s);
 hipError_t t2map_occupancy(int reg, int *blocks_per_cu, size_t lds);
 
 }  // namespace dmi

@@ -1936,6 +1936,26 @@ hipError_t launch_transpose(const double *src, double *dst, int m, int n, int K,
     return hipGetLastError();
 }
 
+template <typename From, typename To>
+__global__ void convert_kernel(const From *src, To *dst, int64_t n) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        dst[t] = (To)src[t];
+}
+
+hipError_t launch_widen(const float *src, double *dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int blocks = (int)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_narrow(const double *src, float *dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int blocks = (int)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(blocks), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_select(const double *image, int64_t V, double thr, int64_t *vox, unsigned int *count, hipStream_t s) {
     int blocks = (int)std::min<int64_t>((V + 255) / 256, 4096);
     if (blocks < 1) blocks = 1;

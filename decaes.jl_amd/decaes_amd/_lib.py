@@ -29,6 +29,7 @@ _dp = C.POINTER(C.c_double)
 EXPORTS = (
     "dmi_create", "dmi_destroy", "dmi_last_error", "dmi_t2map_f64", "dmi_t2map_f64_device", "dmi_epg_basis_set",
     "dmi_t2part_f64", "dmi_t2part_f64_device", "dmi_last_stats", "dmi_synth_mse",
+    "dmi_t2map_f32", "dmi_t2part_f32",
     "dmi_server_create", "dmi_server_destroy", "dmi_server_submit", "dmi_server_wait", "dmi_server_poll",
     "dmi_server_get_stats", "dmi_server_last_error", "dmi_server_request_error",
 )
@@ -47,6 +48,14 @@ class T2mapOptsC(C.Structure):
 
 class T2mapOutC(C.Structure):
     _fields_ = [(k, _dp) for k in ("dist", "gdn", "ggm", "gva", "fnr", "snr", "alpha", "resnorm", "decaycurve", "mu",
+                                   "chi2factor", "decaybasis")]
+
+
+_fp = C.POINTER(C.c_float)
+
+
+class T2mapOutF32C(C.Structure):
+    _fields_ = [(k, _fp) for k in ("dist", "gdn", "ggm", "gva", "fnr", "snr", "alpha", "resnorm", "decaycurve", "mu",
                                    "chi2factor", "decaybasis")]
 
 
@@ -98,6 +107,8 @@ def load() -> C.CDLL:
         L.dmi_last_error.restype = C.c_char_p
         L.dmi_t2map_f64.argtypes = [vp, _dp, C.POINTER(T2mapOptsC), _dp, C.POINTER(T2mapOutC)]
         L.dmi_t2map_f64_device.argtypes = [vp, vp, C.POINTER(T2mapOptsC), vp, C.POINTER(T2mapOutC)]
+        L.dmi_t2map_f32.argtypes = [vp, _fp, C.POINTER(T2mapOptsC), _fp, C.POINTER(T2mapOutF32C)]
+        L.dmi_t2part_f32.argtypes = [vp, _fp, C.POINTER(T2partOptsC), _fp, _fp, _fp, _fp]
         L.dmi_epg_basis_set.argtypes = [vp, C.POINTER(T2mapOptsC), _dp]
         L.dmi_t2part_f64.argtypes = [vp, _dp, C.POINTER(T2partOptsC), _dp, _dp, _dp, _dp]
         L.dmi_t2part_f64_device.argtypes = [vp, vp, C.POINTER(T2partOptsC), vp, vp, vp, vp]

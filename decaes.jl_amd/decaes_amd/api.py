@@ -3,7 +3,9 @@
 `T2mapSEcorr(image, opts)` returns `(maps, dist)` where `maps` holds exactly the keys the reference's
 `convert(Dict{String,Any}, maps)` keeps (src/T2mapSEcorr.jl:21-54) and `dist` is the (X,Y,Z,nT2)
 distribution array; `T2partSEcorr(dist, opts)` returns {"sfr","sgm","mfr","mgm"} (src/T2partSEcorr.jl:40-93).
-Arrays are Fortran-ordered float64, as Julia's column-major arrays. Every voxel runs on the device.
+Arrays are Fortran-ordered, as Julia's column-major arrays. Float64 inputs run the FP64 path; Float32 inputs (the
+reference's T{Float32} methods) return Float32 outputs: the FP64 path's results rounded once. Every voxel runs on
+the device.
 """
 from __future__ import annotations
 
@@ -18,16 +20,21 @@ from . import _lib
 from .options import T2mapOptions, T2partOptions
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 
 
 def _ptr(a: Optional[np.ndarray]):
     if a is None:
         return None
-    return a.ctypes.data_as(_dp)
+    return a.ctypes.data_as(_fp if a.dtype == np.float32 else _dp)
 
 
-def _nan(shape):
-    return np.full(shape, np.nan, order="F")
+def _nan(shape, dtype=np.float64):
+    return np.full(shape, np.nan, order="F", dtype=dtype)
+
+
+def _float_type(a) -> type:
+    return np.float32 if np.asarray(a).dtype == np.float32 else np.float64
 
 
 def t2map_opts_c(o: T2mapOptions) -> _lib.T2mapOptsC:
@@ -81,7 +88,8 @@ def T2mapSEcorr(image: np.ndarray, opts: Optional[T2mapOptions] = None, *, alpha
     `alpha`, if given, is a B1 (flip angle) map as passed through load_B1map! (src/T2mapSEcorr.jl:56-61):
     the flip angle search is skipped and the map is echoed in maps["alpha"].
     """
-    image = np.asfortranarray(image, dtype=np.float64)
+    ft = _float_type(image)
+    image = np.asfortranarray(image, dtype=ft)
     if opts is None:
         opts = T2mapOptions.from_image(image, **kwargs)
     elif kwargs:
@@ -90,32 +98,36 @@ def T2mapSEcorr(image: np.ndarray, opts: Optional[T2mapOptions] = None, *, alpha
     ctx = _lib.context(device)
     shape3 = opts.MatrixSize
     maps = {
-        "echotimes": opts.TE * np.arange(1, opts.nTE + 1, dtype=np.float64),
-        "t2times": opts.T2_component_times(),
+        "echotimes": (opts.TE * np.arange(1, opts.nTE + 1, dtype=np.float64)).astype(ft),
+        "t2times": opts.T2_component_times().astype(ft),
     }
     if opts.SetFlipAngle is None:
-        maps["refangleset"] = opts.flip_angles()
-    maps["decaybasisset"] = decaybasisset(opts, device)
-    out = {k: _nan(shape3) for k in ("gdn", "ggm", "gva", "fnr", "snr", "alpha")}
+        maps["refangleset"] = opts.flip_angles().astype(ft)
+    maps["decaybasisset"] = decaybasisset(opts, device).astype(ft, order="F")
+    out = {k: _nan(shape3, ft) for k in ("gdn", "ggm", "gva", "fnr", "snr", "alpha")}
     if alpha is not None:
-        alpha = np.asfortranarray(alpha, dtype=np.float64)
+        alpha = np.asfortranarray(alpha, dtype=ft)
         assert alpha.shape == shape3
         out["alpha"][...] = alpha
     if opts.SaveResidualNorm:
-        out["resnorm"] = _nan(shape3)
+        out["resnorm"] = _nan(shape3, ft)
     if opts.SaveDecayCurve:
-        out["decaycurve"] = _nan((*shape3, opts.nTE))
+        out["decaycurve"] = _nan((*shape3, opts.nTE), ft)
     if opts.SaveRegParam:
-        out["mu"] = _nan(shape3)
-        out["chi2factor"] = _nan(shape3)
+        out["mu"] = _nan(shape3, ft)
+        out["chi2factor"] = _nan(shape3, ft)
     if opts.SaveNNLSBasis:
-        out["decaybasis"] = (_nan((*shape3, opts.nTE, opts.nT2)) if opts.SetFlipAngle is None
-                             else np.empty((opts.nTE, opts.nT2), order="F"))
-    dist = _nan((*shape3, opts.nT2))
+        out["decaybasis"] = (_nan((*shape3, opts.nTE, opts.nT2), ft) if opts.SetFlipAngle is None
+                             else np.empty((opts.nTE, opts.nT2), order="F", dtype=ft))
+    dist = _nan((*shape3, opts.nT2), ft)
     oc = t2map_opts_c(opts)
-    oo = _lib.T2mapOutC(_ptr(dist), *(_ptr(out.get(k)) for k in (
-        "gdn", "ggm", "gva", "fnr", "snr", "alpha", "resnorm", "decaycurve", "mu", "chi2factor", "decaybasis")))
-    rc = _lib.load().dmi_t2map_f64(ctx.handle, _ptr(image), C.byref(oc), _ptr(alpha), C.byref(oo))
+    keys = ("gdn", "ggm", "gva", "fnr", "snr", "alpha", "resnorm", "decaycurve", "mu", "chi2factor", "decaybasis")
+    if ft is np.float32:
+        oo = _lib.T2mapOutF32C(_ptr(dist), *(_ptr(out.get(k)) for k in keys))
+        rc = _lib.load().dmi_t2map_f32(ctx.handle, _ptr(image), C.byref(oc), _ptr(alpha), C.byref(oo))
+    else:
+        oo = _lib.T2mapOutC(_ptr(dist), *(_ptr(out.get(k)) for k in keys))
+        rc = _lib.load().dmi_t2map_f64(ctx.handle, _ptr(image), C.byref(oc), _ptr(alpha), C.byref(oo))
     _lib.check(rc, ctx.handle)
     if ctx.stats()["voxels"] == 0 and not opts.Silent:
         warnings.warn(f"No voxels found with first-echo signal intensity above threshold {opts.Threshold}.\n"
@@ -126,17 +138,18 @@ def T2mapSEcorr(image: np.ndarray, opts: Optional[T2mapOptions] = None, *, alpha
 
 def T2partSEcorr(dist: np.ndarray, opts: Optional[T2partOptions] = None, *, device: Optional[int] = None, **kwargs):
     """T2partSEcorr(dist, opts) (src/T2partSEcorr.jl:40-71) -> {"sfr", "sgm", "mfr", "mgm"}."""
-    dist = np.asfortranarray(dist, dtype=np.float64)
+    ft = _float_type(dist)
+    dist = np.asfortranarray(dist, dtype=ft)
     if opts is None:
         opts = T2partOptions.from_dist(dist, **kwargs)
     elif kwargs:
         raise TypeError("pass either opts or keyword options, not both")
     assert dist.shape == (*opts.MatrixSize, opts.nT2)
     ctx = _lib.context(device)
-    out = {k: _nan(opts.MatrixSize) for k in ("sfr", "sgm", "mfr", "mgm")}
+    out = {k: _nan(opts.MatrixSize, ft) for k in ("sfr", "sgm", "mfr", "mgm")}
     oc = t2part_opts_c(opts)
-    rc = _lib.load().dmi_t2part_f64(ctx.handle, _ptr(dist), C.byref(oc), _ptr(out["sfr"]), _ptr(out["sgm"]),
-                                    _ptr(out["mfr"]), _ptr(out["mgm"]))
+    fn = _lib.load().dmi_t2part_f32 if ft is np.float32 else _lib.load().dmi_t2part_f64
+    rc = fn(ctx.handle, _ptr(dist), C.byref(oc), _ptr(out["sfr"]), _ptr(out["sgm"]), _ptr(out["mfr"]), _ptr(out["mgm"]))
     _lib.check(rc, ctx.handle)
     return out
 

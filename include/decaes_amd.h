@@ -106,6 +106,20 @@ int dmi_t2map_f64(dmi_ctx *ctx, const double *image, const dmi_t2map_opts *opts,
 int dmi_t2map_f64_device(dmi_ctx *ctx, const double *d_image, const dmi_t2map_opts *opts, const double *d_alpha_in,
                          const dmi_t2map_out *d_out);
 
+/* Float32 arrays: the reference is generic in the element type (T2mapSEcorr(image::Array{T,4}, opts::
+ * T2mapOptions{T}), src/T2mapSEcorr.jl:148-149, and T2partSEcorr(dist::Array{T,4}, ...), src/T2partSEcorr.jl:40).
+ * Here a Float32 image is widened to FP64 on the device, the FP64 path runs, and every output is rounded to
+ * Float32 once, so the results are the FP64 path's rounded to Float32 (the reference instead computes in Float32). */
+typedef struct {
+    float *dist;
+    float *gdn, *ggm, *gva, *fnr, *snr, *alpha;
+    float *resnorm, *decaycurve, *mu, *chi2factor, *decaybasis;
+} dmi_t2map_out_f32;
+int dmi_t2map_f32(dmi_ctx *ctx, const float *image, const dmi_t2map_opts *opts, const float *alpha_in,
+                  const dmi_t2map_out_f32 *out);
+int dmi_t2part_f32(dmi_ctx *ctx, const float *dist, const dmi_t2part_opts *opts, float *sfr, float *sgm, float *mfr,
+                   float *mgm);
+
 /* maps["decaybasisset"] (src/T2mapSEcorr.jl:31-33): nTE x nT2 x nRefAngles, or nTE x nT2 with SetFlipAngle. */
 int dmi_epg_basis_set(dmi_ctx *ctx, const dmi_t2map_opts *opts, double *out);
 

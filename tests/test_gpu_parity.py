@@ -229,3 +229,24 @@ def test_size_limits(dmi, oracle, shape, m, n, K, reg):
         assert np.mean(rel <= 1e-6) >= 0.5
     else:
         assert np.mean(rel <= 1e-6) >= 0.9
+
+
+def test_float32_inputs(dmi):
+    """Float32 arrays: outputs are the FP64 path's on the widened input, rounded once to Float32."""
+    print("[parity] case=float32", flush=True)
+    img, _ = dmi.synth_mse((6, 5, 2), 32, 10e-3, mask_fraction=0.8, seed=21)
+    img32 = img.astype(np.float32, order="F")
+    opts = dmi.T2mapOptions.from_image(img32, TE=10e-3, nT2=40, T2Range=(10e-3, 2.0), Reg="chi2", Chi2Factor=1.02,
+                                       nRefAngles=32, Silent=True, SaveRegParam=True)
+    maps32, dist32 = dmi.T2mapSEcorr(img32, opts)
+    maps64, dist64 = dmi.T2mapSEcorr(img32.astype(np.float64, order="F"), opts)
+    assert dist32.dtype == np.float32 and maps32["gdn"].dtype == np.float32
+    np.testing.assert_array_equal(dist32, dist64.astype(np.float32))
+    for k in ("gdn", "ggm", "gva", "fnr", "snr", "alpha", "mu", "chi2factor"):
+        np.testing.assert_array_equal(maps32[k], maps64[k].astype(np.float32))
+    popts = dmi.T2partOptions.from_t2map(opts, SPWin=(10e-3, 40e-3), MPWin=(40e-3, 2.0))
+    p32 = dmi.T2partSEcorr(dist32, popts)
+    p64 = dmi.T2partSEcorr(dist32.astype(np.float64, order="F"), popts)
+    for k in ("sfr", "sgm", "mfr", "mgm"):
+        assert p32[k].dtype == np.float32
+        np.testing.assert_array_equal(p32[k], p64[k].astype(np.float32))
