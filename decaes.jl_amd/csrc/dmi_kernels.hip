@@ -502,8 +502,11 @@ struct Wave {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const int i = L + k * kWave;
-            u[k] = c[k] / alpha;
-            if (i > ip && i < mr) part = fma(bv[k], u[k], part);
+            u[k] = 0.0;
+            if (i < mr) {  // wave-uniformly false for the third row unless mr > 128
+                u[k] = c[k] / alpha;
+                if (i > ip) part = fma(bv[k], u[k], part);
+            }
         }
         double sm = bip + wsum(part);
         sm *= -tau;
@@ -814,6 +817,7 @@ struct Wave {
             for (;;) {
                 iter += 1;
                 if (iter > max_iter) { status |= kStatusNnlsMaxIter; terminated = true; break; }
+                if (ballot(L < nsetp && zr <= 0) == 0) break;  // alpha == 2 without a candidate: all feasible
                 double t = INFINITY;
                 int ti = INT_MAX;
                 if (L < nsetp && zr <= 0) {
@@ -1244,8 +1248,9 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
     double gnorm = fmax(fabs(lo), fabs(hi));
     // Each lane brackets eigenvalue t with count(a) <= t < count(b), count(x) = #eigenvalues < x, and splits the
     // bracket in four per sweep: three Sturm sequences (LDL^T pivots q_i = (d_i - x) - e_{i-1}^2 / q_{i-1}) run
-    // interleaved, so their divides overlap. The divides must be exact: a pivot near zero amplifies any error in
-    // its quotient into later pivots' signs.
+    // interleaved. Each quotient uses the hardware reciprocal refined by two Newton steps (within an ulp or two of
+    // 1/q; the reciprocal alone is too coarse, since a pivot near zero amplifies a quotient's error into later
+    // pivots' signs). The diagonal and off-diagonal are loaded one step ahead.
     double a = lo, b = hi;
     const int t = L;
     int it = 0;
@@ -1255,18 +1260,29 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
         bool done = !(t < N) || h <= fmax(2.0 * kEps * fmax(fabs(a), fabs(b)), kEps * gnorm) || x2 <= a || x2 >= b;
         if (ballot(!done) == 0) break;
         if (!done) {
-            const double d0 = dv[0];
+            volatile ldbl *dvv = dv, *evv = ev;
+            const double d0 = dvv[0];
             double q1 = d0 - x1, q2 = d0 - x2, q3 = d0 - x3;
             int c1 = q1 < 0, c2 = q2 < 0, c3 = q3 < 0;
+            double dn = dvv[N > 1 ? 1 : 0], en = evv[0];
             for (int i = 1; i < N; ++i) {
-                const double d = dv[i], e = ev[i - 1];
+                const double d = dn, e = en;
+                dn = dvv[i + 1 < N ? i + 1 : i];
+                en = evv[i];
                 const double e2 = e * e, guard = kEps * (fabs(e) + kEps);
                 q1 = (q1 == 0.0) ? guard : q1;
                 q2 = (q2 == 0.0) ? guard : q2;
                 q3 = (q3 == 0.0) ? guard : q3;
-                q1 = (d - x1) - e2 / q1;
-                q2 = (d - x2) - e2 / q2;
-                q3 = (d - x3) - e2 / q3;
+                double r1 = __builtin_amdgcn_rcp(q1), r2 = __builtin_amdgcn_rcp(q2), r3 = __builtin_amdgcn_rcp(q3);
+                r1 = fma(r1, fma(-q1, r1, 1.0), r1);
+                r2 = fma(r2, fma(-q2, r2, 1.0), r2);
+                r3 = fma(r3, fma(-q3, r3, 1.0), r3);
+                r1 = fma(r1, fma(-q1, r1, 1.0), r1);
+                r2 = fma(r2, fma(-q2, r2, 1.0), r2);
+                r3 = fma(r3, fma(-q3, r3, 1.0), r3);
+                q1 = fma(-e2, r1, d - x1);
+                q2 = fma(-e2, r2, d - x2);
+                q3 = fma(-e2, r3, d - x3);
                 c1 += q1 < 0;
                 c2 += q2 < 0;
                 c3 += q3 < 0;
