@@ -1171,26 +1171,47 @@ __device__ __forceinline__ void reg_mdp(Wave &W, double delta, double &mu_out, d
 __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
     const int m = W.m, n = W.n, L = W.L, lda = W.lda;
     DMI_PT0();
-    if (L < n)
-        for (int i = 0; i < m; ++i) W.Aat(i, L) = W.A0[i * n + L];
+    for (int j = 0; j < n; ++j)
+        for (int i = L; i < m; i += kWave) W.Aat(i, j) = W.A0t[(int64_t)j * m + i];
     __syncthreads();
     N = min(m, n);
-    // N x N Gram matrix at rows [m, m + N) of columns [0, N) (lda >= m + n leaves the room)
+    // N x N Gram matrix at rows [m, m + N) of columns [0, N) (lda >= m + n leaves the room), built with FP64
+    // MFMA 16x16x4 blocks: G = A^T A when m >= n, A A^T otherwise. For block (R, C) and k-step s, lane l supplies
+    // X[l % 16][s + l / 16] and Y[s + l / 16][l % 16]; the result lane l, register r is D[4r + l / 16][l % 16].
+    // Only blocks with R <= C are formed and each value is stored at (row, col) and (col, row), so G is exactly
+    // symmetric.
     auto G = [&](int r, int c) -> ldbl & { return W.A[c * lda + m + r]; };
-    if (L < N) {
-        if (m >= n) {
-            for (int c = 0; c < N; ++c) {
-                double s = 0.0;
-                for (int i = 0; i < m; ++i) s = fma(W.Aat(i, L), W.Aat(i, c), s);
-                G(L, c) = s;
+    {
+        typedef double d4 __attribute__((ext_vector_type(4)));
+        const int NB = (N + 15) / 16, K = (m >= n) ? m : n;
+        const int lr = L & 15, lk = L >> 4;
+        for (int R = 0; R < NB; ++R)
+            for (int C = R; C < NB; ++C) {
+                d4 acc = {0.0, 0.0, 0.0, 0.0};
+                const int ra = R * 16 + lr, ca = C * 16 + lr;
+                for (int s0 = 0; s0 < K; s0 += 4) {
+                    const int k = s0 + lk;
+                    double a = 0.0, bb = 0.0;
+                    if (k < K) {
+                        if (m >= n) {
+                            if (ra < N) a = W.Aat(k, ra);
+                            if (ca < N) bb = W.Aat(k, ca);
+                        } else {
+                            if (ra < N) a = W.Aat(ra, k);
+                            if (ca < N) bb = W.Aat(ca, k);
+                        }
+                    }
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    const int row = R * 16 + 4 * rr + lk, col = C * 16 + lr;
+                    if (row < N && col < N && (R < C || row <= col)) {
+                        G(row, col) = acc[rr];
+                        G(col, row) = acc[rr];
+                    }
+                }
             }
-        } else {
-            for (int c = 0; c < N; ++c) {
-                double s = 0.0;
-                for (int j = 0; j < n; ++j) s = fma(W.Aat(L, j), W.Aat(c, j), s);
-                G(L, c) = s;
-            }
-        }
     }
     W.flops += 2ull * N * N * (m >= n ? m : n);
     __syncthreads();
