@@ -237,7 +237,7 @@ int t2map_launch(std::string &err, dmi::T2mapParams p, const int64_t *d_vox, int
     DMI_HIP(err, lb.counter.ensure(sizeof(unsigned) * 4));
     DMI_HIP(err, lb.counters.ensure(sizeof(unsigned long long) * 4));
     DMI_HIP(err, lb.status.ensure(sizeof(int) * 4));
-    const size_t lds = dmi::t2map_lds_bytes(p.m, p.n, p.lda, p.mw);
+    const size_t lds = dmi::t2map_lds_bytes(p.m, p.n, p.lda, p.K);
     int per_cu = 0;
     DMI_HIP(err, dmi::t2map_occupancy(p.reg, &per_cu, lds));
     if (per_cu < 1) return errorf(err, DMI_EUNSUPPORTED, "voxel kernel does not fit on a CU (LDS %zu bytes)", lds);

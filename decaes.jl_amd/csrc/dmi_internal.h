@@ -71,7 +71,7 @@ struct T2partParams {
 };
 
 int t2map_lda(int m, int n);
-size_t t2map_lds_bytes(int m, int n, int lda, int mw);
+size_t t2map_lds_bytes(int m, int n, int lda, int K);  // K: flip-angle grid size of the launch (1 with SetFlipAngle)
 int64_t t2map_slot_doubles(int m, int n);
 
 hipError_t launch_grid_basis(const GridParams &p, hipStream_t s);

@@ -1749,8 +1749,10 @@ __device__ __forceinline__ void voxel_basis(Wave &W, bool fp32, bool general, do
     else voxel_basis_t<double>(W, general, alpha, beta, E1, E2, A0out, A0T);
     // EPG op count of the reference's split recursion: ~13 flops per retained state per echo
     const unsigned long long f = (unsigned long long)W.n * 13ull * ((unsigned long long)W.m * W.m / 4 + W.m);
-    if (fp32) W.flops32 += f;
-    else W.flops += f;
+    // value selects, not a branch on which counter to bump: a pointer select between the two members
+    // would keep both counters in scratch memory instead of registers
+    W.flops32 += fp32 ? f : 0ull;
+    W.flops += fp32 ? 0ull : f;
     __syncthreads();
 }
 
@@ -1769,17 +1771,18 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
     W.A = s; s += (size_t)p.n * p.lda;
     W.b = s; s += p.lda;
     W.zz = s; s += p.lda;
-    W.x = s; s += kMaxT2;
-    W.w = s; s += kMaxT2;
-    W.xf = s; s += kMaxT2;
+    const int ne = (p.n + 1) & ~1;
+    W.x = s; s += ne;
+    W.w = s; s += ne;
+    W.xf = s; s += ne;
     W.b0 = s; s += (p.m + 1) & ~1;
     W.cmu = s; s += kTikhCache;
     W.crn = s; s += kTikhCache;
     W.csemi = s; s += kTikhCache;
     lint *si = (lint *)s;
-    W.idx = si; si += kMaxT2;
-    W.diag = si; si += kMaxT2;
-    W.stk = si;
+    W.idx = si; si += ne;
+    W.diag = si; si += ne;
+    W.stk = si;  // 3 (K + 2) ints: the depth-limited bisection stack holds at most nRefAnglesMin + 1 <= K + 1 boxes
     gdbl *slot = (gdbl *)p.scratch + (int64_t)blockIdx.x * p.slot_stride;
     W.A0s = slot;
     W.A0T = slot + (int64_t)p.m * p.n;
@@ -1943,10 +1946,10 @@ int t2map_lda(int m, int n) {
     return (ld % 4 == 0) ? ld + 2 : ld;
 }
 
-size_t t2map_lds_bytes(int m, int n, int lda, int mw) {
-    (void)mw;
-    size_t d = (size_t)n * lda + 2 * (size_t)lda + 3 * kMaxT2 + (size_t)((m + 1) & ~1) + 3 * kTikhCache;
-    size_t i = 2 * kMaxT2 + 3 * (2 * kMaxAngles + 2);
+size_t t2map_lds_bytes(int m, int n, int lda, int K) {
+    const size_t ne = (size_t)((n + 1) & ~1);
+    size_t d = (size_t)n * lda + 2 * (size_t)lda + 3 * ne + (size_t)((m + 1) & ~1) + 3 * kTikhCache;
+    size_t i = 2 * ne + 3 * ((size_t)K + 2);
     return d * sizeof(double) + i * sizeof(int);
 }
 

@@ -240,7 +240,7 @@ int bind_and_select(dmi_server *S, StreamState &st, Request *r) {
     // while a chunk runs
     const int m = r->o.nTE, n = r->o.nT2;
     const int lda = dmi::t2map_lda(m, n);
-    const size_t lds = dmi::t2map_lds_bytes(m, n, lda, m + n);
+    const size_t lds = dmi::t2map_lds_bytes(m, n, lda, fixed ? 1 : r->o.nRefAngles);  // as t2map_params sets K
     int grid = 0;
     for (int reg = DMI_REG_NONE; reg <= DMI_REG_MDP; ++reg) {
         int per_cu = 0;
