@@ -476,12 +476,17 @@ struct Wave {
     __device__ __forceinline__ double construct_apply_householder(int ip, int jp, int mr) {
         if (ip >= mr) return 0.0;
         const ldbl *cp = A + jp * lda;
-        double c[3], bv[3];
+        ldbl *ci = A + ip * lda;
+        ldbl *cj = A + jp * lda;
+        // t: column ip, which moves to column jp when the pivot column is swapped into place (read up front;
+        // nothing writes it before the swap)
+        double c[3], bv[3], t[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const int i = L + k * kWave;
             c[k] = (i < mr) ? cp[i] : 0.0;
             bv[k] = (i < mr) ? b[i] : 0.0;
+            t[k] = (ip != jp && i < mr) ? ci[i] : 0.0;
         }
         double alpha = rfl(cp[ip]);
         double part = 0.0;
@@ -515,24 +520,16 @@ struct Wave {
         // b1 / A1 > 0 (A1 = -beta is nonzero): a sign test, with the divide kept for the quotient-underflow case
         const bool pos = (b1 > 0 && A1 > 0) || (b1 < 0 && A1 < 0);
         if (!(fabs(b1) >= 1e-290 ? pos : (b1 / A1 > 0))) return -1.0;
-        ldbl *ci = A + ip * lda;
-        ldbl *cj = A + jp * lda;
         if (ip < mr - 1) {
+            // branch-free per lane: column ip takes (c above the pivot, A1, u below), column jp takes the old
+            // column ip, and b takes (b unchanged, b1, b + sm u)
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const int i = L + k * kWave;
                 if (i >= mr) continue;
-                if (i < ip) {
-                    if (ip != jp) { const double t = ci[i]; ci[i] = c[k]; cj[i] = t; }
-                } else if (i == ip) {
-                    b[ip] = b1;
-                    if (ip != jp) { const double t = ci[ip]; ci[ip] = A1; cj[ip] = t; }
-                    else ci[ip] = A1;
-                } else {
-                    if (ip != jp) { const double t = ci[i]; ci[i] = u[k]; cj[i] = t; }
-                    else ci[i] = u[k];
-                    b[i] = fma(sm, u[k], bv[k]);
-                }
+                ci[i] = (i < ip) ? c[k] : ((i == ip) ? A1 : u[k]);
+                if (ip != jp) cj[i] = t[k];
+                b[i] = (i < ip) ? bv[k] : ((i == ip) ? b1 : fma(sm, u[k], bv[k]));
             }
             flops += 3ull * (mr - ip);
         } else {
@@ -541,7 +538,7 @@ struct Wave {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const int i = L + k * kWave;
-                    if (i < mr) { const double t = ci[i]; ci[i] = c[k]; cj[i] = t; }
+                    if (i < mr) { ci[i] = c[k]; cj[i] = t[k]; }
                 }
         }
         __syncthreads();
@@ -567,12 +564,18 @@ struct Wave {
             if (h) s1 = cj[lo] * cu[lo];
             if (t) s2 = cj[hi - 1] * cu[hi - 1];
             int q = q0;
-            for (; q + 2 <= q1; q += 2) {
+            // four row pairs per step (all eight loads issued before the first use), then two, then one;
+            // accumulator k always takes the same row of a step, as a plain two-pair loop would
+            for (; q + 4 <= q1; q += 4) {
                 const dv2 a0 = vj[q], u0 = vu[q], a1 = vj[q + 1], u1 = vu[q + 1];
-                s0 = fma(a0.x, u0.x, s0);
-                s1 = fma(a0.y, u0.y, s1);
-                s2 = fma(a1.x, u1.x, s2);
-                s3 = fma(a1.y, u1.y, s3);
+                const dv2 a2 = vj[q + 2], u2 = vu[q + 2], a3 = vj[q + 3], u3 = vu[q + 3];
+                s0 = fma(a0.x, u0.x, s0); s1 = fma(a0.y, u0.y, s1); s2 = fma(a1.x, u1.x, s2); s3 = fma(a1.y, u1.y, s3);
+                s0 = fma(a2.x, u2.x, s0); s1 = fma(a2.y, u2.y, s1); s2 = fma(a3.x, u3.x, s2); s3 = fma(a3.y, u3.y, s3);
+            }
+            if (q + 2 <= q1) {
+                const dv2 a0 = vj[q], u0 = vu[q], a1 = vj[q + 1], u1 = vu[q + 1];
+                s0 = fma(a0.x, u0.x, s0); s1 = fma(a0.y, u0.y, s1); s2 = fma(a1.x, u1.x, s2); s3 = fma(a1.y, u1.y, s3);
+                q += 2;
             }
             if (q < q1) {
                 const dv2 a0 = vj[q], u0 = vu[q];
@@ -636,12 +639,18 @@ struct Wave {
             if (t) s1 = cj[hi - 1] * b[hi - 1];
             int q = bs >> 1;
             const int q1 = be >> 1;
-            for (; q + 2 <= q1; q += 2) {
+            // four row pairs per step (all eight loads issued before the first use), then two, then one;
+            // accumulator k always takes the same row of a step, as a plain two-pair loop would
+            for (; q + 4 <= q1; q += 4) {
                 const dv2 a0 = vj[q], u0 = vb[q], a1 = vj[q + 1], u1 = vb[q + 1];
-                s0 = fma(a0.x, u0.x, s0);
-                s1 = fma(a0.y, u0.y, s1);
-                s2 = fma(a1.x, u1.x, s2);
-                s3 = fma(a1.y, u1.y, s3);
+                const dv2 a2 = vj[q + 2], u2 = vb[q + 2], a3 = vj[q + 3], u3 = vb[q + 3];
+                s0 = fma(a0.x, u0.x, s0); s1 = fma(a0.y, u0.y, s1); s2 = fma(a1.x, u1.x, s2); s3 = fma(a1.y, u1.y, s3);
+                s0 = fma(a2.x, u2.x, s0); s1 = fma(a2.y, u2.y, s1); s2 = fma(a3.x, u3.x, s2); s3 = fma(a3.y, u3.y, s3);
+            }
+            if (q + 2 <= q1) {
+                const dv2 a0 = vj[q], u0 = vb[q], a1 = vj[q + 1], u1 = vb[q + 1];
+                s0 = fma(a0.x, u0.x, s0); s1 = fma(a0.y, u0.y, s1); s2 = fma(a1.x, u1.x, s2); s3 = fma(a1.y, u1.y, s3);
+                q += 2;
             }
             if (q < q1) {
                 const dv2 a0 = vj[q], u0 = vb[q];
