@@ -460,9 +460,8 @@ struct Wave {
     lint *idx, *diag, *stk;
     int lda, mw, m, n, L;
     // per-workgroup global scratch
-    gdbl *A0s, *A0T, *cx, *lc;
+    gdbl *A0s, *cx, *lc;
     cgdbl *A0;   // basis used by the regularization solves (A0s or the fixed basis), row-major
-    cgdbl *A0t;  // the same basis with columns contiguous
     int ccur;
     unsigned long long flops, flops32, solves;
     int status;
@@ -1180,8 +1179,8 @@ __device__ __forceinline__ void reg_mdp(Wave &W, double delta, double &mu_out, d
 __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
     const int m = W.m, n = W.n, L = W.L, lda = W.lda;
     DMI_PT0();
-    for (int j = 0; j < n; ++j)
-        for (int i = L; i < m; i += kWave) W.Aat(i, j) = W.A0t[(int64_t)j * m + i];
+    if (L < n)
+        for (int i = 0; i < m; ++i) W.Aat(i, L) = W.A0[(int64_t)i * n + L];
     __syncthreads();
     N = min(m, n);
     // N x N Gram matrix at rows [m, m + N) of columns [0, N) (lda >= m + n leaves the room), built with FP64
@@ -1737,7 +1736,7 @@ struct FlipSearch {
 // Basis at one flip angle: n EPG curves, two per wave (32-lane groups) when the echo window fits.
 template <typename T>
 __device__ __forceinline__ void voxel_basis_t(Wave &W, bool general, double alpha, double beta, double E1, cgdbl *E2,
-                                              gdbl *A0out, gdbl *A0T) {
+                                              gdbl *A0out) {
     const int m = W.m, n = W.n, L = W.L;
     if (m / 2 + 2 <= 32) {
         const int h = L >> 5, s = L & 31;
@@ -1745,17 +1744,17 @@ __device__ __forceinline__ void voxel_basis_t(Wave &W, bool general, double alph
             int j = jp + h;
             bool valid = j < n;
             int jj = valid ? j : n - 1;
-            epg_curve<32, T>(general, alpha, beta, E1, E2[jj], m, s, valid, A0out + jj, nullptr, n, A0T + (int64_t)jj * m);
+            epg_curve<32, T>(general, alpha, beta, E1, E2[jj], m, s, valid, A0out + jj, nullptr, n, nullptr);
         }
     } else {
         for (int j = 0; j < n; ++j)
-            epg_curve<64, T>(general, alpha, beta, E1, E2[j], m, L, true, A0out + j, nullptr, n, A0T + (int64_t)j * m);
+            epg_curve<64, T>(general, alpha, beta, E1, E2[j], m, L, true, A0out + j, nullptr, n, nullptr);
     }
 }
 __device__ __forceinline__ void voxel_basis(Wave &W, bool fp32, bool general, double alpha, double beta, double E1,
-                                            cgdbl *E2, gdbl *A0out, gdbl *A0T) {
-    if (fp32) voxel_basis_t<float>(W, general, alpha, beta, E1, E2, A0out, A0T);
-    else voxel_basis_t<double>(W, general, alpha, beta, E1, E2, A0out, A0T);
+                                            cgdbl *E2, gdbl *A0out) {
+    if (fp32) voxel_basis_t<float>(W, general, alpha, beta, E1, E2, A0out);
+    else voxel_basis_t<double>(W, general, alpha, beta, E1, E2, A0out);
     // EPG op count of the reference's split recursion: ~13 flops per retained state per echo
     const unsigned long long f = (unsigned long long)W.n * 13ull * ((unsigned long long)W.m * W.m / 4 + W.m);
     // value selects, not a branch on which counter to bump: a pointer select between the two members
@@ -1793,9 +1792,10 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
     W.diag = si; si += ne;
     W.stk = si;  // 3 (K + 2) ints: the depth-limited bisection stack holds at most nRefAnglesMin + 1 <= K + 1 boxes
     gdbl *slot = (gdbl *)p.scratch + (int64_t)blockIdx.x * p.slot_stride;
+    // slot: the voxel's row-major basis (one copy, so the slots of the resident workgroups stay within L2), the
+    // Tikhonov cache solutions and the L-curve cache
     W.A0s = slot;
-    W.A0T = slot + (int64_t)p.m * p.n;
-    W.cx = W.A0T + (int64_t)p.m * p.n;
+    W.cx = slot + (int64_t)p.m * p.n;
     W.lc = W.cx + (int64_t)kTikhCache * p.n;
     W.flops = 0;
     W.flops32 = 0;
@@ -1827,7 +1827,6 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
         if (p.mode == kModeFixed) {
             alpha_v = p.setflip;
             W.A0 = (cgdbl *)p.Afixed;
-            W.A0t = (cgdbl *)p.AfixedT;
         } else {
             if (p.mode == kModeB1) {
                 alpha_v = p.alpha_in[I];
@@ -1841,10 +1840,9 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
                 DMI_PT1(kPhFlipSearch);
             }
             DMI_PT0();
-            voxel_basis(W, p.fp32 != 0, p.general != 0, alpha_v, p.beta, p.E1, (cgdbl *)p.E2, W.A0s, W.A0T);
+            voxel_basis(W, p.fp32 != 0, p.general != 0, alpha_v, p.beta, p.E1, (cgdbl *)p.E2, W.A0s);
             DMI_PT1(kPhVoxelBasis);
             W.A0 = W.A0s;
-            W.A0t = W.A0T;
         }
         // T2_distribution! (src/T2mapSEcorr.jl:475-505)
         double muv = 0.0, chi2v = 1.0;
@@ -1868,7 +1866,7 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
         double pr2 = 0.0, prs = 0.0;
         for (int i = L; i < m; i += kWave) {
             double fit = 0.0;
-            for (int j = 0; j < n; ++j) fit += W.A0t[j * m + i] * W.xf[j];
+            for (int j = 0; j < n; ++j) fit += W.A0[i * n + j] * W.xf[j];
             double bi = W.b0[i] * max_signal;
             double res = fit - bi;
             if (i == L) res0 = res; else res1 = res;
@@ -1963,7 +1961,7 @@ size_t t2map_lds_bytes(int m, int n, int lda, int K) {
 }
 
 int64_t t2map_slot_doubles(int m, int n) {
-    int64_t s = 2 * (int64_t)m * n + (int64_t)kTikhCache * n + 19 * (int64_t)kLcurveCap;
+    int64_t s = (int64_t)m * n + (int64_t)kTikhCache * n + 19 * (int64_t)kLcurveCap;
     return (s + 31) / 32 * 32;
 }
 
