@@ -889,7 +889,7 @@ struct Wave {
         bool pos = false;
         if (L < n) {
             double wj = 0.0;
-#pragma unroll 8
+#pragma unroll 16
             for (int i = 0; i < m; ++i) {
                 double a = A0g[i * n + L];
                 wj += a * zz[i];
@@ -1247,15 +1247,37 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
         if (L < N) vv[L] = vr;
         if (L == 0) ev[k] = ah;
         __syncthreads();
+        // p = G v over the trailing rows: four partial sums, so the FMA chain is a quarter as long
         double pr = 0.0;
-        if (L > k && L < N)
-            for (int c = k + 1; c < N; ++c) pr = fma(G(L, c), vv[c], pr);
+        if (L > k && L < N) {
+            double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+            int c = k + 1;
+            for (; c + 4 <= N; c += 4) {
+                p0 = fma(G(L, c), vv[c], p0);
+                p1 = fma(G(L, c + 1), vv[c + 1], p1);
+                p2 = fma(G(L, c + 2), vv[c + 2], p2);
+                p3 = fma(G(L, c + 3), vv[c + 3], p3);
+            }
+            for (; c < N; ++c) p0 = fma(G(L, c), vv[c], p0);
+            pr = (p0 + p1) + (p2 + p3);
+        }
         double Kq = wsum(vr * pr);
         double qr = pr - Kq * vr;
         if (L < N) qv[L] = qr;
         __syncthreads();
-        if (L > k && L < N)
-            for (int c = k + 1; c < N; ++c) G(L, c) -= 2.0 * (vr * qv[c] + qr * vv[c]);
+        // rank-2 update, four columns per step with every load of the step ahead of its stores (the loads of
+        // qv / vv could alias the stores to G as far as the compiler knows, so a plain loop would serialize)
+        if (L > k && L < N) {
+            int c = k + 1;
+            for (; c + 4 <= N; c += 4) {
+                double g[4], q[4], v[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) { g[r] = G(L, c + r); q[r] = qv[c + r]; v[r] = vv[c + r]; }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) G(L, c + r) = g[r] - 2.0 * (vr * q[r] + qr * v[r]);
+            }
+            for (; c < N; ++c) G(L, c) -= 2.0 * (vr * qv[c] + qr * vv[c]);
+        }
         W.flops += 8ull * (N - k) * (N - k);
         __syncthreads();
     }
