@@ -261,14 +261,23 @@ template <> struct Num<DualF> {
     }
     static __device__ __forceinline__ double val(DualF x) { return (double)x.v; }
 };
-template <int LPC> __device__ __forceinline__ float shup(float a) { return __shfl_up(a, 1, LPC); }
-template <int LPC> __device__ __forceinline__ float shdn(float a) { return __shfl_down(a, 1, LPC); }
-template <int LPC> __device__ __forceinline__ DualF shup(DualF a) { return {__shfl_up(a.v, 1, LPC), __shfl_up(a.d, 1, LPC)}; }
-template <int LPC> __device__ __forceinline__ DualF shdn(DualF a) { return {__shfl_down(a.v, 1, LPC), __shfl_down(a.d, 1, LPC)}; }
-template <int LPC> __device__ __forceinline__ double shup(double a) { return __shfl_up(a, 1, LPC); }
-template <int LPC> __device__ __forceinline__ double shdn(double a) { return __shfl_down(a, 1, LPC); }
-template <int LPC> __device__ __forceinline__ Dual shup(Dual a) { return {__shfl_up(a.v, 1, LPC), __shfl_up(a.d, 1, LPC)}; }
-template <int LPC> __device__ __forceinline__ Dual shdn(Dual a) { return {__shfl_down(a.v, 1, LPC), __shfl_down(a.d, 1, LPC)}; }
+// Phase-state shifts between neighbouring lanes: DPP whole-wave shifts (wave_shr:1, lane i <- lane i - 1, and
+// wave_shl:1, lane i <- lane i + 1), a few cycles instead of an LDS permute. Every lane of the wave is active
+// here. At a group boundary (LPC = 32: lanes 31 | 32; lane 0 and lane 63 read 0) the value crossing over is
+// discarded by the caller: state s = 0 takes its own Fbar and state LPC - 1 takes Fbar = 0.
+constexpr int kWaveShr1 = 0x138;
+constexpr int kWaveShl1 = 0x130;
+template <int CTRL> __device__ __forceinline__ float dpp_f(float a) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(a), CTRL, 0xf, 0xf, true));
+}
+template <int LPC> __device__ __forceinline__ float shup(float a) { return dpp_f<kWaveShr1>(a); }
+template <int LPC> __device__ __forceinline__ float shdn(float a) { return dpp_f<kWaveShl1>(a); }
+template <int LPC> __device__ __forceinline__ DualF shup(DualF a) { return {dpp_f<kWaveShr1>(a.v), dpp_f<kWaveShr1>(a.d)}; }
+template <int LPC> __device__ __forceinline__ DualF shdn(DualF a) { return {dpp_f<kWaveShl1>(a.v), dpp_f<kWaveShl1>(a.d)}; }
+template <int LPC> __device__ __forceinline__ double shup(double a) { return dpp_d<kWaveShr1>(a); }
+template <int LPC> __device__ __forceinline__ double shdn(double a) { return dpp_d<kWaveShl1>(a); }
+template <int LPC> __device__ __forceinline__ Dual shup(Dual a) { return {dpp_d<kWaveShr1>(a.v), dpp_d<kWaveShr1>(a.d)}; }
+template <int LPC> __device__ __forceinline__ Dual shdn(Dual a) { return {dpp_d<kWaveShl1>(a.v), dpp_d<kWaveShl1>(a.d)}; }
 
 // One EPG decay curve per LPC-lane group, lane s = phase state s+1. Each echo maps state k = (F, Fbar, Z)
 // to F_{k+1}, Fbar_{k-1} and Z_k; the state-1 Fbar result becomes the new F_1 and is the echo. The
