@@ -706,16 +706,23 @@ struct Wave {
 
     // Remove coefficient imv from set P by Givens rotations and column shifts, then drop any further
     // nonpositive coefficients (:729-779 / :965-1015). pidx / xp: the column and coefficient at position L.
+    // The rows below the diagonal of an active position hold zeros in the reference; here they are left as they
+    // are (the NNLS loop does not clear the reflector it leaves there) and stand for zeros: the removed column is
+    // cleared below its diagonal before the rotations fill it in, and the rotation at rows (i - 1, i) skips the
+    // other active positions j < i, whose two entries are zero before and after it in the reference (below the
+    // diagonal, or, for j = i - 1 > imv, the zero the previous rotation wrote).
     __device__ __forceinline__ void nnls_delete(int &imv, int &nsetp, int mrows, int &pidx, double &xp) {
         for (;;) {
             if (L == imv) xp = 0.0;
+            for (int r = imv + 1 + L; r < mrows; r += kWave) Aat(r, imv) = 0.0;
+            __syncthreads();
             if (imv != nsetp - 1) {
                 for (int i = imv + 1; i <= nsetp - 1; ++i) {
                     double aa = rfl(Aat(i - 1, i)), bb = rfl(Aat(i, i));
                     double sig = hypot(aa, bb), cc = aa / sig, ss = bb / sig;
                     __syncthreads();
                     const int j = L;
-                    if (j < n) {
+                    if (j < n && (j >= i || j == imv)) {
                         if (j == i) { Aat(i - 1, i) = sig; Aat(i, i) = 0.0; }
                         else {
                             double u = Aat(i - 1, j), v = Aat(i, j);
@@ -819,9 +826,9 @@ struct Wave {
                 DMI_PT1(kPhApply);
             }
             DMI_LAP(kPhSwapApply);
-            for (int i = nsetp + L; i < mr; i += kWave) Aat(i, nsetp - 1) = 0.0;
-            if (L == 0) w[nsetp - 1] = 0.0;
-            __syncthreads();
+            // The reference now zeroes the new pivot column below its diagonal and its dual; neither is read
+            // again while the position is active (nnls_delete treats those rows as zeros, and duals are read
+            // and recomputed only for inactive positions), so both are left as they are.
             double zr = (L < nsetp) ? b[L] : 0.0;
             DMI_LAP(kPhPostApply);
             {
