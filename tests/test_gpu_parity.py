@@ -250,3 +250,48 @@ def test_float32_inputs(dmi):
     for k in ("sfr", "sgm", "mfr", "mgm"):
         assert p32[k].dtype == np.float32
         np.testing.assert_array_equal(p32[k], p64[k].astype(np.float32))
+
+
+def test_full_volume_sampled(dmi, oracle):
+    """The 240x240x48 gcv volume (1.66M voxels above threshold) through the host-array entry point: every output
+    against the oracle on a seeded sample of in-mask voxels spread over the whole volume (so over the whole work
+    queue of the persistent grid), plus size-independent invariants on all voxels."""
+    shape, m, TE, n, T2R = (240, 240, 48), 48, 10e-3, 40, (10e-3, 2.0)
+    print(f"[parity] full volume {shape} nTE={m} gcv", flush=True)
+    img, nin = dmi.synth_mse(shape, m, TE, ncomp=3, mask_fraction=0.6, seed=3)
+    opts = dmi.T2mapOptions.from_image(img, TE=TE, nT2=n, T2Range=T2R, Reg="gcv", nRefAngles=32,
+                                       SaveRegParam=True, Silent=True)
+    maps, dist = dmi.T2mapSEcorr(img, opts)
+    inmask = img[..., 0] > 0
+    assert int(inmask.sum()) == nin
+    # NaN exactly outside the threshold mask; finite, positive totals inside; gdn is the distribution's sum
+    assert np.array_equal(~np.isnan(maps["gdn"]), inmask)
+    assert np.array_equal(~np.isnan(dist[..., 0]), inmask)
+    g = maps["gdn"][inmask]
+    assert np.all(np.isfinite(g)) and np.all(g > 0)
+    d = dist[inmask]
+    assert np.all(d >= 0)
+    assert np.max(np.abs(d.sum(axis=1) - g) / g) <= 1e-12
+    a = maps["alpha"][inmask]
+    assert np.all((a >= 50.0 - 1e-9) & (a <= 180.0 + 1e-9))
+    parts = dmi.T2partSEcorr(dist, dmi.T2partOptions.from_dist(dist, T2Range=T2R, SPWin=(T2R[0], 40e-3),
+                                                              MPWin=(40e-3, T2R[1]), Silent=True))
+    sfr = parts["sfr"][inmask]
+    assert np.all((sfr >= 0) & (sfr <= 1))
+    # sampled parity against the oracle
+    rng = np.random.default_rng(1234)
+    lin = np.flatnonzero(inmask.ravel(order="F"))
+    vox = np.sort(rng.choice(lin, size=3000, replace=False))
+    oo = oracle.make_t2map_opts(shape, m, TE, n, T2R, "gcv", nRefAngles=32, nRefAnglesMin=opts.nRefAnglesMin)
+    omaps, odist = oracle.t2map_subset(img, oo, vox, nthreads=min(os.cpu_count() or 1, 16))
+    sel = np.zeros(inmask.size, bool)
+    sel[vox] = True
+    sel = sel.reshape(inmask.shape, order="F")
+    rel = np.linalg.norm(dist[sel] - odist[sel], axis=1) / np.linalg.norm(odist[sel], axis=1)
+    da = np.abs(maps["alpha"][sel] - omaps["alpha"][sel])
+    po = oracle.t2part(odist, T2R, (T2R[0], 40e-3), (40e-3, T2R[1]))
+    mwf_err = np.abs(parts["sfr"][sel] - po["sfr"][sel])
+    s = _summary("c2_full_volume_sampled", sel, rel, da, mwf_err, maps, omaps)
+    assert np.max(da) <= 1e-2, s
+    assert s["frac_ok"] >= 0.97, s
+    assert np.mean(np.abs(maps["mu"][sel] - omaps["mu"][sel]) <= 1e-6 * omaps["mu"][sel]) >= 0.97, s
