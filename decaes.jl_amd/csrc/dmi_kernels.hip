@@ -678,11 +678,12 @@ struct Wave {
     // q0 = z y: Markstein's correction, the correctly rounded quotient when y is the correctly rounded
     // reciprocal. Every lane evaluates it on its own z (only lane j's is the final z_j at step j), so the
     // sequential chain per column is fma, mul, fma, fma, readlane; column j - 2 is loaded during step j.
+    // Lane i's z is final from step i on (later steps only update lanes below j), so its quotient is formed once
+    // more after the loop instead of being captured at step i.
     __device__ __forceinline__ double tri_solve(int k, double zr) {
         if (k <= 0) return zr;
         const double aL = (L < k) ? Aat(L, L) : 1.0;
         const double rL = 1.0 / aL;
-        double zf = zr;
         // rows L < 64 of any column j < n stay inside the LDS allocation (b, zz, x, w follow the matrix)
         // volatile: keeps each load at its place in the pipeline instead of being moved next to its use
         volatile ldbl *Av = A;
@@ -694,14 +695,14 @@ struct Wave {
             const double qL = fma(fma(-q0, aL, zr), rL, q0);
             const double qj = readlane(qL, j);
             zr = fma((L < j) ? c0 : 0.0, -qj, zr);
-            zf = (L == j) ? qj : zf;
             if (j == 0) break;
             --j;
             c0 = c1;
             c1 = c2;
         }
         flops += (unsigned long long)k * k;
-        return (L < k) ? zf : zr;
+        const double q0 = zr * rL;
+        return (L < k) ? fma(fma(-q0, aL, zr), rL, q0) : zr;
     }
 
     // Remove coefficient imv from set P by Givens rotations and column shifts, then drop any further
