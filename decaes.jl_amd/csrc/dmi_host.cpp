@@ -220,7 +220,6 @@ dmi::T2mapParams t2map_params(const dmi_t2map_opts *o, const Grids &g, const dou
     p.AsT = fixed ? nullptr : g.AsT.as<double>();
     p.dAsT = fixed ? nullptr : g.dAsT.as<double>();
     p.Afixed = fixed ? g.Afixed.as<double>() : nullptr;
-    p.AfixedT = fixed ? g.AfixedT.as<double>() : nullptr;
     p.alpha_in = d_alpha;
     p.dist = d->dist; p.gdn = d->gdn; p.ggm = d->ggm; p.gva = d->gva; p.fnr = d->fnr; p.snr = d->snr; p.alpha = d->alpha;
     p.resnorm = o->SaveResidualNorm ? d->resnorm : nullptr;

@@ -51,7 +51,6 @@ struct T2mapParams {
     const double *AsT;    // K x n x m (columns contiguous)
     const double *dAsT;   // d/dalpha, K x n x m
     const double *Afixed; // m x n row-major (SetFlipAngle)
-    const double *AfixedT;// n x m
     const double *alpha_in;
     double *scratch;      // per-workgroup slots
     int64_t slot_stride;  // doubles per slot
