@@ -1665,9 +1665,13 @@ struct FlipSearch {
         double part = 0.0;
         for (int i = W.L; i < m; i += kWave) {
             double ax = 0.0, dax = 0.0;
+            // every column, without a branch on x_j > 0, so the loads of all columns are issued together: an
+            // inactive column has x_j = 0 exactly and fma(0, a, s) = s for finite a
+#pragma unroll 8
             for (int j = 0; j < n; ++j) {
-                double xj = W.x[j];
-                if (xj > 0) { ax = fma(xj, AT[j * m + i], ax); dax = fma(xj, dAT[j * m + i], dax); }
+                const double xj = W.x[j];
+                ax = fma(xj, AT[j * m + i], ax);
+                dax = fma(xj, dAT[j * m + i], dax);
             }
             ax -= W.b0[i];
             part += dax * ax;
