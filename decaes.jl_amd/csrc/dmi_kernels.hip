@@ -465,9 +465,11 @@ struct Wave {
     // LDS. A is column-major with column stride lda (even, lda = 2 mod 4, lda >= mw rounded up to even):
     // a lane that owns a column streams it with 16-byte loads, and 16-byte loads of 16 consecutive lanes
     // at stride lda hit distinct banks. Every LDS double array starts 16-byte aligned.
-    ldbl *A, *b, *zz, *x, *w, *xf, *b0, *cmu, *crn, *csemi;
+    ldbl *A, *b, *zz, *x, *xf, *b0, *cmu, *crn, *csemi;
     lint *idx, *diag, *stk;
     int lda, mw, m, n, L;
+    // the dual w_j of position j = L (the reference's w), in a register: only this lane writes or reads it
+    double wl;
     // per-workgroup global scratch
     gdbl *A0s, *cx, *lc;
     cgdbl *A0;   // basis used by the regularization solves (A0s or the fixed basis), row-major
@@ -628,7 +630,7 @@ struct Wave {
                 w1 = fma(a, b[hi - 1], w1);
                 cj[hi - 1] = a;
             }
-            w[j] = w0 + w1;
+            wl = w0 + w1;
         }
         flops += 6ull * (m1 - j1) * (n - j1 - 1);
         __syncthreads();
@@ -665,9 +667,9 @@ struct Wave {
                 s0 = fma(a0.x, u0.x, s0);
                 s1 = fma(a0.y, u0.y, s1);
             }
-            w[j] = (s0 + s1) + (s2 + s3);
+            wl = (s0 + s1) + (s2 + s3);
         } else if (j >= j1 && j < n) {
-            w[j] = 0.0;
+            wl = 0.0;
         }
         flops += 2ull * (m1 - j1) * (n - j1);
         __syncthreads();
@@ -781,7 +783,7 @@ struct Wave {
             double tau = 0.0;
             bool term = false;
             DMI_PT0();
-            double wv = (L >= nsetp && L < n) ? w[L] : 0.0;
+            double wv = (L >= nsetp && L < n) ? wl : 0.0;
             for (;;) {
                 double v = (wv > 0) ? wv : 0.0;
                 int wi = (v > 0) ? L : INT_MAX;
@@ -801,11 +803,8 @@ struct Wave {
                     DMI_PT1(kPhConstruct);
                 }
                 if (tau >= 0) break;
-                if (L == jmax) wv = 0.0;
-                if (L == 0) {
-                    w[jmax] = 0.0;
-                    if (tikh && mr < M) Aat(mr, jmax) = 0.0;
-                }
+                if (L == jmax) { wv = 0.0; wl = 0.0; }
+                if (L == 0 && tikh && mr < M) Aat(mr, jmax) = 0.0;
                 __syncthreads();
             }
             DMI_LAP(kPhSelect);
@@ -912,7 +911,7 @@ struct Wave {
                 wj += a * zz[i];
                 Aat(i, L) = a;
             }
-            if (L < n - 1) { w[L] = wj; pos = !(wj <= 0); }
+            if (L < n - 1) { wl = wj; pos = !(wj <= 0); }
             x[L] = 0.0;
             idx[L] = L;
             diag[L] = 0;
@@ -920,7 +919,7 @@ struct Wave {
                 for (int i = m; i < M; ++i) Aat(i, L) = 0.0;
         }
         unsigned long long bal = ballot(pos);
-        if (L == n - 1) w[L] = (bal == 0) ? 1.0 : 0.0;
+        if (L == n - 1) wl = (bal == 0) ? 1.0 : 0.0;
         for (int i = L; i < m; i += kWave) b[i] = b0[i];
         if (tikh)
             for (int i = m + L; i < M; i += kWave) b[i] = 0.0;
@@ -1824,7 +1823,6 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
     W.zz = s; s += p.lda;
     const int ne = (p.n + 1) & ~1;
     W.x = s; s += ne;
-    W.w = s; s += ne;
     W.xf = s; s += ne;
     W.b0 = s; s += (p.m + 1) & ~1;
     W.cmu = s; s += kTikhCache;
@@ -1998,7 +1996,7 @@ int t2map_lda(int m, int n) {
 
 size_t t2map_lds_bytes(int m, int n, int lda, int K) {
     const size_t ne = (size_t)((n + 1) & ~1);
-    size_t d = (size_t)n * lda + 2 * (size_t)lda + 3 * ne + (size_t)((m + 1) & ~1) + 3 * kTikhCache;
+    size_t d = (size_t)n * lda + 2 * (size_t)lda + 2 * ne + (size_t)((m + 1) & ~1) + 3 * kTikhCache;
     size_t i = 2 * ne + 3 * ((size_t)K + 2);
     return d * sizeof(double) + i * sizeof(int);
 }
