@@ -441,7 +441,11 @@ __shared__ unsigned long long s_phase[2 * kPhCount];
     do { if (lane_id() == 0) for (int _k = 0; _k < 2 * kPhCount; ++_k) s_phase[_k] = 0; } while (0)
 #define DMI_PHASE_END() \
     do { if (lane_id() == 0) for (int _k = 0; _k < 2 * kPhCount; ++_k) atomicAdd(&g_phase[_k], s_phase[_k]); } while (0)
+// Tikhonov solves by the number of Tikhonov rows they appended (columns that entered the active set)
+__device__ unsigned long long g_tikh_rows[kMaxT2 + 1];
+#define DMI_TIKH_ROWS(k) do { if (lane_id() == 0) atomicAdd(&g_tikh_rows[(k)], 1ull); } while (0)
 #else
+#define DMI_TIKH_ROWS(k) do {} while (0)
 #define DMI_PT0() do {} while (0)
 #define DMI_PT1(k) do {} while (0)
 #define DMI_LAP(k) do {} while (0)
@@ -886,6 +890,7 @@ struct Wave {
         for (int i = nsetp + L; i < Mlim; i += kWave) { double bi = b[i]; part = fma(bi, bi, part); }
         rnorm_out = sqrt(wsum(part));
         nsetp_out = nsetp;
+        if (tikh) DMI_TIKH_ROWS(__popcll(dmask));
     }
 
     // NNLSProblem solve! with the reference warm start (src/lsqnonneg.jl:30-84), or the Tikhonov form
@@ -1986,7 +1991,13 @@ extern "C" int dmi_phase_read(unsigned long long *out) {
 }
 extern "C" int dmi_phase_reset() {
     unsigned long long z[2 * kPhCount] = {0};
+    unsigned long long zt[kMaxT2 + 1] = {0};
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(g_tikh_rows), zt, sizeof zt);
+    if (e != hipSuccess) return (int)e;
     return (int)hipMemcpyToSymbol(HIP_SYMBOL(g_phase), z, sizeof z);
+}
+extern "C" int dmi_tikh_rows_read(unsigned long long *out) {
+    return (int)hipMemcpyFromSymbol(out, HIP_SYMBOL(g_tikh_rows), sizeof(g_tikh_rows));
 }
 #endif
 int t2map_lda(int m, int n) {

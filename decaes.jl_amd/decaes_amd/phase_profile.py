@@ -34,6 +34,7 @@ def run(names):
     _lib.LIB_PATH = _build.PHASE_LIB
     L = _lib.load()
     L.dmi_phase_read.argtypes = [C.POINTER(C.c_ulonglong)]
+    L.dmi_tikh_rows_read.argtypes = [C.POINTER(C.c_ulonglong)]
     import decaes_amd as d
 
     for wl in names:
@@ -59,6 +60,20 @@ def run(names):
                 continue
             print(f"   {name:>15s} {buf[k] / nin:13.0f} {100 * buf[k] / max(voxel, 1):7.1f}% "
                   f"{buf[NPH + k] / nin:12.2f} {buf[k] / buf[NPH + k]:12.0f}", flush=True)
+        rows = (C.c_ulonglong * 65)()
+        L.dmi_tikh_rows_read(rows)
+        tot = sum(rows)
+        if tot:
+            cum, p50, p99, mx = 0, None, None, max(i for i in range(65) if rows[i])
+            for i in range(65):
+                cum += rows[i]
+                if p50 is None and cum >= 0.5 * tot:
+                    p50 = i
+                if p99 is None and cum >= 0.99 * tot:
+                    p99 = i
+            over = ", ".join(f">{c}: {sum(rows[c + 1:]) / tot:.2e}" for c in (16, 24, 32, 40, 48) if c < n)
+            print(f"   Tikhonov rows appended per solve ({tot} solves): median {p50}, p99 {p99}, max {mx}; "
+                  f"fraction of solves {over}", flush=True)
 
 
 if __name__ == "__main__":
