@@ -2,9 +2,12 @@
 
 One step = T2mapSEcorr over the whole 240x240x48 48-echo volume (flip-angle search, EPG bases, GCV-regularized
 NNLS, maps) followed by T2partSEcorr on its distribution, with the image and all outputs resident in HBM.
-value = processed (above-threshold) voxels per second. Prints one JSON line.
+value = processed (above-threshold) voxels per second. Prints one JSON line, which also carries the MWF error of
+the run against the plain FP64 path (config.mwf_err_vs_fp64; 0 for the FP64 path itself), the p50/p99 latency of a
+short served stream through the request loop (served), the roofline object and the CPU baseline.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3] [--precision fp64|mixed]
+                    [--no-cpu-baseline] [--no-served]
     python bench.py --copy-probe BYTES    # plain device copy of known size, the HBM-counter calibration run
     python bench.py --serve [--qps 2,8,32] [--requests 16] [--max-slices 8]   # served stream: p50/p99 latency
     python bench.py --batch [--volumes 64]   # batched uneven workload of seeded volumes through the request loop
@@ -12,6 +15,7 @@ value = processed (above-threshold) voxels per second. Prints one JSON line.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -97,10 +101,10 @@ def copy_probe(nbytes: int):
     print(json.dumps({"copy_bytes": a.numel() * 4, "GBps_read_plus_write": 2 * a.numel() * 4 / dt / 1e9}), flush=True)
 
 
-def serve(args):
+def serve_levels(args) -> list:
     """Served stream through the request loop: Poisson arrivals at each QPS of requests of 1..max_slices slices of
-    240x240 (nTE in {32, 48, 56}, Reg in {chi2, gcv, lcurve}, T2part on each), one JSON line with p50/p99 latency
-    and voxels/s per QPS."""
+    240x240 (nTE in {32, 48, 56}, Reg in {chi2, gcv, lcurve}, T2part on each); p50/p99 latency and voxels/s per
+    QPS."""
     import threading
 
     import numpy as np
@@ -162,6 +166,12 @@ def serve(args):
                        "voxels_per_s_wall": st["voxels"] / wall, "failed": st["failed"]})
         log(f"[serve] qps {qps}: p50 {st['p50_ms']:.0f} ms p99 {st['p99_ms']:.0f} ms, {st['voxels']} voxels, "
             f"{levels[-1]['voxels_per_s_busy']:.0f} voxels/s busy")
+    return levels
+
+
+def serve(args):
+    """One JSON line with the served stream's p50/p99 latency and voxels/s per QPS (serve_levels)."""
+    levels = serve_levels(args)
     print(json.dumps({"metric": "served latency (T2map+T2part requests)", "unit": "ms", "levels": levels,
                       "config": {"requests_per_level": args.requests, "slices": f"1..{args.max_slices} of 240x240",
                                  "nTE": [32, 48, 56], "Reg": ["chi2", "gcv", "lcurve"], "seed": args.seed},
@@ -225,6 +235,7 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-served", action="store_true", help="skip the short served stream reported beside the value")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
     ap.add_argument("--serve", action="store_true")
@@ -327,6 +338,29 @@ def main():
                      "kernel": f"t2map_kernel<{dmi.api._lib.REG[w['Reg']]}>", "kernel_ms": kernel_ms,
                      "flops_per_launch": flops},
     }
+    # the metric's accuracy clause: MWF (sfr) of this run against the plain FP64 path on the same volume
+    if args.precision == "fp64":
+        res["config"]["mwf_err_vs_fp64"] = 0.0
+    else:
+        log("[bench] plain FP64 path on the same volume for the MWF error")
+        o64 = dataclasses.replace(opts, precision="fp64")
+        out64 = alloc_t2map_outputs(o64, dev)
+        p64 = {k: torch.full((Z, Y, X), float("nan"), dtype=torch.float64, device=dev) for k in pout}
+        t2map_device(d_img, o64, out64, ctx=ctx)
+        t2part_device(out64["dist"], popts, p64, ctx=ctx)
+        a, b = pout["sfr"].cpu().numpy(), p64["sfr"].cpu().numpy()
+        ok = np.isfinite(b)
+        res["config"]["mwf_err_vs_fp64"] = float(np.max(np.abs(a[ok] - b[ok]))) if ok.any() else None
+    if not args.no_served:
+        # served p50/p99 (the metric's latency clause): a short Poisson stream at 8 requests/s through the request loop
+        log("[bench] served stream (8 requests of 1-4 slices of 240x240 at QPS 8)")
+        try:
+            lv = serve_levels(argparse.Namespace(qps="8", requests=8, max_slices=4, seed=7))[0]
+            res["served"] = {"qps": lv["qps"], "requests": lv["requests"], "p50_ms": lv["p50_ms"],
+                             "p99_ms": lv["p99_ms"], "voxels_per_s_busy": lv["voxels_per_s_busy"],
+                             "failed": lv["failed"], "stream": "1-4 slices of 240x240, nTE 32/48/56, chi2/gcv/lcurve"}
+        except Exception as e:  # the throughput line is still printed
+            res["served"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle on host cores)")
         res["cpu_baseline"] = cpu_baseline(img, dict(w, **{"nTE": w["nTE"]}), args.cpu_seconds)
