@@ -413,7 +413,7 @@ enum Phase : int {
     kPhConstruct = 1, kPhApply = 2, kPhTriSolve = 3, kPhDelete = 4, kPhDual = 5, kPhInit = 6, kPhNnls = 7,
     kPhGram = 8, kPhTridiag = 9, kPhFlipSearch = 10, kPhVoxelBasis = 11, kPhSturm = 12, kPhGramEigs = 13,
     kPhRegTotal = 14, kPhVoxel = 15, kPhSelect = 16, kPhFeasible = 17, kPhPostApply = 18,
-    kPhSwapApply = 19, kPhTriSolveOuter = 20,
+    kPhSwapApply = 19, kPhTriSolveOuter = 20, kPhEigBracket = 21,
     kPhCount = 24
 };
 #ifdef DMI_PHASE_PROFILE
@@ -1318,13 +1318,93 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
     lo = wmind(lo);
     hi = wmaxd(hi);
     double gnorm = fmax(fabs(lo), fabs(hi));
+    const double eg = kEps * gnorm;
+    const int t = L;
+    // One Sturm sequence at x (the same pivots as the sweeps below): count(x) = #eigenvalues < x, and
+    // g(x) = sum_i q_i' / q_i = sum_j 1 / (x - lambda_j), the logarithmic derivative of det(T - x I), from the
+    // pivots' derivatives q_i' = -1 + e_{i-1}^2 q_{i-1}' / q_{i-1}^2.
+    auto chain = [&](double x, int &count, double &g) {
+        volatile ldbl *dvv = dv, *evv = ev;
+        double q = dvv[0] - x, qp = -1.0, gs = 0.0;
+        int c = q < 0;
+        double dn = dvv[N > 1 ? 1 : 0], en = evv[0];
+        for (int i = 1; i < N; ++i) {
+            const double d = dn, e = en;
+            dn = dvv[i + 1 < N ? i + 1 : i];
+            en = evv[i];
+            const double e2 = e * e, guard = kEps * (fabs(e) + kEps);
+            q = (q == 0.0) ? guard : q;
+            double r = __builtin_amdgcn_rcp(q);
+            r = fma(r, fma(-q, r, 1.0), r);
+            r = fma(r, fma(-q, r, 1.0), r);
+            gs = fma(qp, r, gs);
+            qp = fma(e2 * qp, r * r, -1.0);
+            q = fma(-e2, r, d - x);
+            c += q < 0;
+        }
+        count = c;
+        g = gs + qp / ((q == 0.0) ? eg : q);
+    };
+    // Initial brackets: lane l counts the eigenvalues below threshold x_l (x_0 above the spectrum, x_1..x_61
+    // log-spaced from gnorm down to eps * gnorm, x_62 = 0, x_63 = -4 eps * gnorm); eigenvalue t lies between the two
+    // thresholds where the count passes t, a relative width of 0.8 however small it is.
+    const double hi1 = hi + 4.0 * eg + 4.0 * kEps * fabs(hi), lo1 = lo - 4.0 * eg - 4.0 * kEps * fabs(lo);
+    double xl;
+    if (L == 0) xl = hi1;
+    else if (L <= 61) xl = gnorm * exp2(-52.0 * (double)L / 61.0);
+    else if (L == 62) xl = 0.0;
+    else xl = -4.0 * eg;
+    int cl;
+    double gl;
+    chain(xl, cl, gl);
+    unsigned long long mymask = 0ull;
+    for (int tt = 0; tt < N; ++tt) {
+        const unsigned long long msk = ballot(cl > tt);
+        if (L == tt) mymask = msk;
+    }
+    const int Lt = __popcll(mymask);
+    const bool contiguous = mymask == ((Lt == 64) ? ~0ull : ((1ull << Lt) - 1ull));
+    const double xb = __shfl(xl, max(Lt - 1, 0), kWave), xa = __shfl(xl, min(Lt, 63), kWave);
+    double a = lo1, b = hi1;
+    if (contiguous && Lt > 0) {
+        b = xb;
+        a = (Lt < 64) ? xa : lo1;
+    }
+    // Safeguarded Newton on det(T - x I) inside the bracket: each step counts at its point and moves the bracket end
+    // on that side; a step that leaves the bracket is replaced by its midpoint (geometric for a positive bracket
+    // wider than 2x), and once a step is below the tolerance the next point is placed just past the root so the
+    // count closes the bracket. Lanes still open after 30 steps finish in the four-way sweeps below.
+    bool ndone = !(t < N);
+    double x = (a > 0.0 && b > 2.0 * a) ? sqrt(a * b) : 0.5 * (a + b);
+    int itn = 0;
+    for (; itn < 30; ++itn) {
+        if (!ndone) {
+            const double tol = fmax(2.0 * kEps * fmax(fabs(a), fabs(b)), eg);
+            ndone = (b - a) <= tol || !(x > a && x < b);
+        }
+        if (ballot(!ndone) == 0) break;
+        int c;
+        double g;
+        chain(x, c, g);
+        if (!ndone) {
+            const bool below = !(c > t);  // x is below eigenvalue t
+            if (below) a = x;
+            else b = x;
+            const double tol = fmax(2.0 * kEps * fmax(fabs(a), fabs(b)), eg);
+            double xn = x - 1.0 / g;
+            if (isfinite(xn) && fabs(xn - x) <= tol) xn = below ? xn + 0.5 * tol : xn - 0.5 * tol;
+            if (!(isfinite(xn) && xn > a && xn < b))
+                xn = (a > 0.0 && b > 2.0 * a) ? sqrt(a * b) : 0.5 * (a + b);
+            x = xn;
+        }
+    }
+    W.flops += 6ull * N * N * (itn + 1);  // one derivative-carrying Sturm sequence per step
+    DMI_LAP(kPhEigBracket);
     // Each lane brackets eigenvalue t with count(a) <= t < count(b), count(x) = #eigenvalues < x, and splits the
     // bracket in four per sweep: three Sturm sequences (LDL^T pivots q_i = (d_i - x) - e_{i-1}^2 / q_{i-1}) run
     // interleaved. Each quotient uses the hardware reciprocal refined by two Newton steps (within an ulp or two of
     // 1/q; the reciprocal alone is too coarse, since a pivot near zero amplifies a quotient's error into later
     // pivots' signs). The diagonal and off-diagonal are loaded one step ahead.
-    double a = lo, b = hi;
-    const int t = L;
     int it = 0;
     for (; it < 100; ++it) {
         const double h = b - a;

@@ -25,7 +25,7 @@ WORKLOADS = {
 PHASES = [(1, "construct"), (2, "apply"), (3, "trisolve"), (4, "delete"), (5, "dual"), (6, "init"), (7, "nnls"),
           (16, " select+constr"), (18, " post-apply"), (17, " feasibility"),
           (10, "flip search"), (11, "voxel basis"), (13, "gram eigs"), (8, " gram matrix"), (9, " tridiagonal"),
-          (12, " sturm"), (14, "regularization"),
+          (21, " eig brackets"), (12, " sturm"), (14, "regularization"),
           (15, "reg + save")]
 NPH = 24
 
