@@ -444,8 +444,12 @@ __shared__ unsigned long long s_phase[2 * kPhCount];
 // Tikhonov solves by the number of Tikhonov rows they appended (columns that entered the active set)
 __device__ unsigned long long g_tikh_rows[kMaxT2 + 1];
 #define DMI_TIKH_ROWS(k) do { if (lane_id() == 0) atomicAdd(&g_tikh_rows[(k)], 1ull); } while (0)
+// voxels by the number of Laguerre steps their GCV eigenvalue brackets took
+__device__ unsigned long long g_eig_steps[32];
+#define DMI_EIG_STEPS(k) do { if (lane_id() == 0) atomicAdd(&g_eig_steps[(k) < 31 ? (k) : 31], 1ull); } while (0)
 #else
 #define DMI_TIKH_ROWS(k) do {} while (0)
+#define DMI_EIG_STEPS(k) do {} while (0)
 #define DMI_PT0() do {} while (0)
 #define DMI_PT1(k) do {} while (0)
 #define DMI_LAP(k) do {} while (0)
@@ -1323,16 +1327,15 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
     // One Sturm sequence at x (the same pivots as the sweeps below): count(x) = #eigenvalues < x, and
     // g(x) = sum_i q_i' / q_i = sum_j 1 / (x - lambda_j), the logarithmic derivative of det(T - x I), from the
     // pivots' derivatives q_i' = -1 + e_{i-1}^2 q_{i-1}' / q_{i-1}^2.
+    // The diagonal, e^2 and the zero-pivot guard of step i come from lane i's registers by readlane (uniform
+    // values, no LDS traffic in the chain).
+    const double dL = (L < N) ? dv[L] : 0.0, eL = (L < N - 1) ? ev[L] : 0.0;
+    const double e2L = eL * eL, gL = kEps * (fabs(eL) + kEps);
     auto chain = [&](double x, int &count, double &g) {
-        volatile ldbl *dvv = dv, *evv = ev;
-        double q = dvv[0] - x, qp = -1.0, gs = 0.0;
+        double q = readlane(dL, 0) - x, qp = -1.0, gs = 0.0;
         int c = q < 0;
-        double dn = dvv[N > 1 ? 1 : 0], en = evv[0];
         for (int i = 1; i < N; ++i) {
-            const double d = dn, e = en;
-            dn = dvv[i + 1 < N ? i + 1 : i];
-            en = evv[i];
-            const double e2 = e * e, guard = kEps * (fabs(e) + kEps);
+            const double d = readlane(dL, i), e2 = readlane(e2L, i - 1), guard = readlane(gL, i - 1);
             q = (q == 0.0) ? guard : q;
             double r = __builtin_amdgcn_rcp(q);
             r = fma(r, fma(-q, r, 1.0), r);
@@ -1370,36 +1373,77 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
         b = xb;
         a = (Lt < 64) ? xa : lo1;
     }
-    // Safeguarded Newton on det(T - x I) inside the bracket: each step counts at its point and moves the bracket end
-    // on that side; a step that leaves the bracket is replaced by its midpoint (geometric for a positive bracket
-    // wider than 2x), and once a step is below the tolerance the next point is placed just past the root so the
-    // count closes the bracket. Lanes still open after 30 steps finish in the four-way sweeps below.
-    bool ndone = !(t < N);
-    double x = (a > 0.0 && b > 2.0 * a) ? sqrt(a * b) : 0.5 * (a + b);
+    // Two Sturm sequences per step: at x1 with the pivots' first and second derivatives (g = sum 1 / (x - lambda_j)
+    // and g' = -sum 1 / (x - lambda_j)^2), and at x2 for its count only.
+    auto chain2 = [&](double x1, double x2, int &c1, double &g, double &gd, int &c2) {
+        const double d0 = readlane(dL, 0);
+        double q1 = d0 - x1, q2 = d0 - x2, qp = -1.0, qpp = 0.0, gs = 0.0, hs = 0.0;
+        int k1 = q1 < 0, k2 = q2 < 0;
+        for (int i = 1; i < N; ++i) {
+            const double d = readlane(dL, i), e2 = readlane(e2L, i - 1), guard = readlane(gL, i - 1);
+            q1 = (q1 == 0.0) ? guard : q1;
+            q2 = (q2 == 0.0) ? guard : q2;
+            double r1 = __builtin_amdgcn_rcp(q1), r2 = __builtin_amdgcn_rcp(q2);
+            r1 = fma(r1, fma(-q1, r1, 1.0), r1);
+            r2 = fma(r2, fma(-q2, r2, 1.0), r2);
+            r1 = fma(r1, fma(-q1, r1, 1.0), r1);
+            r2 = fma(r2, fma(-q2, r2, 1.0), r2);
+            const double r1s = r1 * r1, qp2 = qp * qp;
+            gs = fma(qp, r1, gs);                                // q'/q
+            hs = fma(fma(qpp, q1, -qp2), r1s, hs);               // (q'' q - q'^2) / q^2
+            qpp = e2 * fma(qpp, q1, -2.0 * qp2) * (r1s * r1);   // q_i'' = e^2 (q'' q - 2 q'^2) / q^3
+            qp = fma(e2 * qp, r1s, -1.0);                        // q_i' = -1 + e^2 q' / q^2
+            q1 = fma(-e2, r1, d - x1);
+            q2 = fma(-e2, r2, d - x2);
+            k1 += q1 < 0;
+            k2 += q2 < 0;
+        }
+        const double qz = (q1 == 0.0) ? eg : q1, rz = 1.0 / qz;
+        c1 = k1;
+        c2 = k2;
+        g = gs + qp * rz;
+        gd = hs + fma(qpp, qz, -(qp * qp)) * (rz * rz);
+    };
+    auto mid = [](double lo_, double hi_) { return (lo_ > 0.0 && hi_ > 2.0 * lo_) ? sqrt(lo_ * hi_) : 0.5 * (lo_ + hi_); };
+    // Laguerre's iteration for the real-rooted det(T - x I) (cubic convergence to the root next to x, on either
+    // side) inside the bracket, with a midpoint count in the same step: each step moves the bracket ends by both
+    // counts, and a Laguerre point that leaves the bracket is replaced by the midpoint (geometric for a positive
+    // bracket wider than 2x). Once a step is below the tolerance, the next step counts at the root +- tol, which
+    // closes the bracket around it. Lanes still open after 30 steps finish in the four-way sweeps below.
+    bool conv = false;
+    double x = mid(a, b), xn = x;
     int itn = 0;
     for (; itn < 30; ++itn) {
-        if (!ndone) {
-            const double tol = fmax(2.0 * kEps * fmax(fabs(a), fabs(b)), eg);
-            ndone = (b - a) <= tol || !(x > a && x < b);
+        bool open = t < N;
+        double p1 = 0.0, p2 = 0.0;
+        if (open) {
+            const double tol = fmax(2.0 * kEps * fmax(fabs(a), fabs(b)), eg), mm = mid(a, b);
+            open = (b - a) > tol && mm > a && mm < b;
+            p1 = conv ? xn - tol : x;
+            p2 = conv ? xn + tol : mm;
+            if (!(p1 > a && p1 < b)) p1 = mm;
+            if (!(p2 > a && p2 < b)) p2 = mm;
         }
-        if (ballot(!ndone) == 0) break;
-        int c;
-        double g;
-        chain(x, c, g);
-        if (!ndone) {
-            const bool below = !(c > t);  // x is below eigenvalue t
-            if (below) a = x;
-            else b = x;
+        if (ballot(open) == 0) break;
+        int c1, c2;
+        double g, gd;
+        chain2(p1, p2, c1, g, gd, c2);
+        if (open) {
+            if (c1 > t) b = fmin(b, p1);
+            else a = fmax(a, p1);
+            if (c2 > t) b = fmin(b, p2);
+            else a = fmax(a, p2);
             const double tol = fmax(2.0 * kEps * fmax(fabs(a), fabs(b)), eg);
-            double xn = x - 1.0 / g;
-            if (isfinite(xn) && fabs(xn - x) <= tol) xn = below ? xn + 0.5 * tol : xn - 0.5 * tol;
-            if (!(isfinite(xn) && xn > a && xn < b))
-                xn = (a > 0.0 && b > 2.0 * a) ? sqrt(a * b) : 0.5 * (a + b);
-            x = xn;
+            const double disc = (double)(N - 1) * ((double)N * (-gd) - g * g);
+            const double sq = sqrt(fmax(disc, 0.0)), den = (g >= 0.0) ? g + sq : g - sq;
+            xn = p1 - (double)N / den;
+            conv = isfinite(xn) && fabs(xn - p1) <= tol;
+            x = (isfinite(xn) && xn > a && xn < b) ? xn : mid(a, b);
         }
     }
-    W.flops += 6ull * N * N * (itn + 1);  // one derivative-carrying Sturm sequence per step
+    W.flops += 5ull * N * N * (1 + 2 * itn);  // the threshold sequence and two sequences per step
     DMI_LAP(kPhEigBracket);
+    DMI_EIG_STEPS(itn);
     // Each lane brackets eigenvalue t with count(a) <= t < count(b), count(x) = #eigenvalues < x, and splits the
     // bracket in four per sweep: three Sturm sequences (LDL^T pivots q_i = (d_i - x) - e_{i-1}^2 / q_{i-1}) run
     // interleaved. Each quotient uses the hardware reciprocal refined by two Newton steps (within an ulp or two of
@@ -2074,10 +2118,16 @@ extern "C" int dmi_phase_reset() {
     unsigned long long zt[kMaxT2 + 1] = {0};
     hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(g_tikh_rows), zt, sizeof zt);
     if (e != hipSuccess) return (int)e;
+    unsigned long long ze[32] = {0};
+    e = hipMemcpyToSymbol(HIP_SYMBOL(g_eig_steps), ze, sizeof ze);
+    if (e != hipSuccess) return (int)e;
     return (int)hipMemcpyToSymbol(HIP_SYMBOL(g_phase), z, sizeof z);
 }
 extern "C" int dmi_tikh_rows_read(unsigned long long *out) {
     return (int)hipMemcpyFromSymbol(out, HIP_SYMBOL(g_tikh_rows), sizeof(g_tikh_rows));
+}
+extern "C" int dmi_eig_steps_read(unsigned long long *out) {
+    return (int)hipMemcpyFromSymbol(out, HIP_SYMBOL(g_eig_steps), sizeof(g_eig_steps));
 }
 #endif
 int t2map_lda(int m, int n) {

@@ -35,6 +35,7 @@ def run(names):
     L = _lib.load()
     L.dmi_phase_read.argtypes = [C.POINTER(C.c_ulonglong)]
     L.dmi_tikh_rows_read.argtypes = [C.POINTER(C.c_ulonglong)]
+    L.dmi_eig_steps_read.argtypes = [C.POINTER(C.c_ulonglong)]
     import decaes_amd as d
 
     for wl in names:
@@ -60,6 +61,11 @@ def run(names):
                 continue
             print(f"   {name:>15s} {buf[k] / nin:13.0f} {100 * buf[k] / max(voxel, 1):7.1f}% "
                   f"{buf[NPH + k] / nin:12.2f} {buf[k] / buf[NPH + k]:12.0f}", flush=True)
+        steps = (C.c_ulonglong * 32)()
+        L.dmi_eig_steps_read(steps)
+        if sum(steps):
+            hist = ", ".join(f"{k}: {steps[k]}" for k in range(32) if steps[k])
+            print(f"   GCV eigenvalue Laguerre steps per voxel: {hist}", flush=True)
         rows = (C.c_ulonglong * 65)()
         L.dmi_tikh_rows_read(rows)
         tot = sum(rows)
