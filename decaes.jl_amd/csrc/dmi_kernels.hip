@@ -13,6 +13,7 @@
 // Arithmetic follows the reference statement by statement, with fma() where DECAES writes muladd or sits
 // under @muladd (src/NNLS.jl:47, src/EPGdecaycurve.jl:858-1024); built with -ffp-contract=off.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include <cfloat>
 #include <climits>
@@ -314,6 +315,49 @@ __device__ __forceinline__ void epg_const(double alpha_deg, double E1, double E2
             const T r = tabs(m0 * echo);
             N::put(r, out_v, out_d, i * ostride);
             if (outT) outT[i] = N::val(r);
+        }
+    }
+}
+
+// Two constant-flip curves per lane group at once (FP64 values only: the voxel's final basis), each with exactly
+// the operations of epg_const<LPC, double>: the two recursions are independent, so their dependent chains
+// interleave and the lane's issue slots are filled by one while the other waits.
+template <int LPC>
+__device__ __forceinline__ void epg_const_pair(double alpha_deg, double E1, double E2a, double E2b, int ETL, int s,
+                                               bool store_a, bool store_b, gdbl *out_a, gdbl *out_b, int64_t ostride) {
+    double sn, cs;
+    sincos(alpha_deg * kDeg2Rad, &sn, &cs);
+    const double m0 = dsind(alpha_deg / 2);
+    const double E1sq = E1 * E1, d = cs * E1sq;
+    const double E2ha = (E2a * E2a) / 2, E2hb = (E2b * E2b) / 2;
+    const double aa = E2ha, ba = cs * E2ha, ca = sn * (E1 * E2a), cpa = (-ca) / 2.0, mca = -ca;
+    const double ab = E2hb, bb = cs * E2hb, cb = sn * (E1 * E2b), cpb = (-cb) / 2.0, mcb = -cb;
+    double Fa = (s == 0) ? 1.0 : 0.0, Fba = 0.0, Za = 0.0;
+    double Fb = Fa, Fbb = 0.0, Zb = 0.0;
+    for (int i = 0; i < ETL; ++i) {
+        const double Ca = Fa + Fba, Sa = Fa - Fba, Cb = Fb + Fbb, Sb = Fb - Fbb;
+        double ea, eb;
+        if (i < ETL - 1) {
+            const double Cpa = aa * Ca, Spa = ba * Sa, Cpb = ab * Cb, Spb = bb * Sb;
+            const double Fra = fma(ca, Za, Cpa + Spa), Fbra = fma(mca, Za, Cpa - Spa), Zra = fma(cpa, Sa, d * Za);
+            const double Frb = fma(cb, Zb, Cpb + Spb), Fbrb = fma(mcb, Zb, Cpb - Spb), Zrb = fma(cpb, Sb, d * Zb);
+            const double Fupa = shup<LPC>(Fra), Fbdna = shdn<LPC>(Fbra);
+            const double Fupb = shup<LPC>(Frb), Fbdnb = shdn<LPC>(Fbrb);
+            ea = Fbra;
+            eb = Fbrb;
+            Fa = (s == 0) ? Fbra : Fupa;
+            Fb = (s == 0) ? Fbrb : Fupb;
+            Fba = (s == LPC - 1) ? 0.0 : Fbdna;
+            Fbb = (s == LPC - 1) ? 0.0 : Fbdnb;
+            Za = Zra;
+            Zb = Zrb;
+        } else {
+            ea = fma(mca, Za, fma(aa, Ca, -(ba * Sa)));
+            eb = fma(mcb, Zb, fma(ab, Cb, -(bb * Sb)));
+        }
+        if (s == 0) {
+            if (store_a) out_a[i * ostride] = fabs(m0 * ea);
+            if (store_b) out_b[i * ostride] = fabs(m0 * eb);
         }
     }
 }
@@ -1909,6 +1953,25 @@ template <typename T>
 __device__ __forceinline__ void voxel_basis_t(Wave &W, bool general, double alpha, double beta, double E1, cgdbl *E2,
                                               gdbl *A0out) {
     const int m = W.m, n = W.n, L = W.L;
+    if constexpr (std::is_same<T, double>::value) {
+        if (!general) {
+            // two curves per lane group and pass (epg_const_pair)
+            if (m / 2 + 2 <= 32) {
+                const int h = L >> 5, s = L & 31;
+                for (int jp = 0; jp < n; jp += 4) {
+                    const int j1 = jp + h, j2 = jp + 2 + h;
+                    const int jj1 = j1 < n ? j1 : n - 1, jj2 = j2 < n ? j2 : n - 1;
+                    epg_const_pair<32>(alpha, E1, E2[jj1], E2[jj2], m, s, j1 < n, j2 < n, A0out + jj1, A0out + jj2, n);
+                }
+            } else {
+                for (int j = 0; j < n; j += 2) {
+                    const int j2 = j + 1 < n ? j + 1 : n - 1;
+                    epg_const_pair<64>(alpha, E1, E2[j], E2[j2], m, L, true, j + 1 < n, A0out + j, A0out + j2, n);
+                }
+            }
+            return;
+        }
+    }
     if (m / 2 + 2 <= 32) {
         const int h = L >> 5, s = L & 31;
         for (int jp = 0; jp < n; jp += 2) {
