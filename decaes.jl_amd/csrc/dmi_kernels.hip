@@ -731,7 +731,7 @@ struct Wave {
     // zz[j] / A[j,j] is formed in lane j from its own reciprocal y = 1/A[j,j] as q = q0 + (z - q0 A[j,j]) y with
     // q0 = z y: Markstein's correction, the correctly rounded quotient when y is the correctly rounded
     // reciprocal. Every lane evaluates it on its own z (only lane j's is the final z_j at step j), so the
-    // sequential chain per column is fma, mul, fma, fma, readlane; column j - 2 is loaded during step j.
+    // sequential chain per column is fma, mul, fma, fma, readlane; column j - 3 is loaded during step j.
     // Lane i's z is final from step i on (later steps only update lanes below j), so its quotient is formed once
     // more after the loop instead of being captured at step i.
     __device__ __forceinline__ double tri_solve(int k, double zr) {
@@ -741,19 +741,32 @@ struct Wave {
         // rows L < 64 of any column j < n stay inside the LDS allocation (b, zz, x, w follow the matrix)
         // volatile: keeps each load at its place in the pipeline instead of being moved next to its use
         volatile ldbl *Av = A;
-        double c0 = Av[(k - 1) * lda + L];
-        double c1 = Av[max(k - 2, 0) * lda + L];
-        for (int j = k - 1;;) {
-            const double c2 = Av[max(j - 2, 0) * lda + L];
+        // three column registers in rotation, unrolled so that no register is copied: the column of step j - 3
+        // is loaded as soon as step j has used its register, two steps before it is needed
+        double ca = Av[(k - 1) * lda + L];
+        double cb = Av[max(k - 2, 0) * lda + L];
+        double cc = Av[max(k - 3, 0) * lda + L];
+        auto step = [&](int j, double c) {
             const double q0 = zr * rL;
             const double qL = fma(fma(-q0, aL, zr), rL, q0);
             const double qj = readlane(qL, j);
-            zr = fma((L < j) ? c0 : 0.0, -qj, zr);
-            if (j == 0) break;
-            --j;
-            c0 = c1;
-            c1 = c2;
+            zr = fma((L < j) ? c : 0.0, -qj, zr);
+        };
+        // whole blocks of three steps (one exit, so every pass around the loop has the same loads in flight),
+        // then the last one or two steps with the columns already in ca and cb
+        int j = k - 1;
+        for (; j >= 2; j -= 3) {
+            step(j, ca);
+            ca = Av[max(j - 3, 0) * lda + L];
+            __builtin_amdgcn_sched_barrier(0);
+            step(j - 1, cb);
+            cb = Av[max(j - 4, 0) * lda + L];
+            __builtin_amdgcn_sched_barrier(0);
+            step(j - 2, cc);
+            cc = Av[max(j - 5, 0) * lda + L];
         }
+        if (j >= 0) step(j, ca);
+        if (j >= 1) step(j - 1, cb);
         flops += (unsigned long long)k * k;
         const double q0 = zr * rL;
         return (L < k) ? fma(fma(-q0, aL, zr), rL, q0) : zr;
