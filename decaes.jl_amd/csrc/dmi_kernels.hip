@@ -626,13 +626,24 @@ struct Wave {
             if (h) s1 = cj[lo] * cu[lo];
             if (t) s2 = cj[hi - 1] * cu[hi - 1];
             int q = q0;
-            // four row pairs per step (all eight loads issued before the first use), then two, then one;
+            // eight row pairs per step (all sixteen loads issued before the first use), then four, two, one;
             // accumulator k always takes the same row of a step, as a plain two-pair loop would
-            for (; q + 4 <= q1; q += 4) {
+            for (; q + 8 <= q1; q += 8) {
+                dv2 a[8], u[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) { a[k] = vj[q + k]; u[k] = vu[q + k]; }
+#pragma unroll
+                for (int k = 0; k < 8; k += 2) {
+                    s0 = fma(a[k].x, u[k].x, s0); s1 = fma(a[k].y, u[k].y, s1);
+                    s2 = fma(a[k + 1].x, u[k + 1].x, s2); s3 = fma(a[k + 1].y, u[k + 1].y, s3);
+                }
+            }
+            if (q + 4 <= q1) {
                 const dv2 a0 = vj[q], u0 = vu[q], a1 = vj[q + 1], u1 = vu[q + 1];
                 const dv2 a2 = vj[q + 2], u2 = vu[q + 2], a3 = vj[q + 3], u3 = vu[q + 3];
                 s0 = fma(a0.x, u0.x, s0); s1 = fma(a0.y, u0.y, s1); s2 = fma(a1.x, u1.x, s2); s3 = fma(a1.y, u1.y, s3);
                 s0 = fma(a2.x, u2.x, s0); s1 = fma(a2.y, u2.y, s1); s2 = fma(a3.x, u3.x, s2); s3 = fma(a3.y, u3.y, s3);
+                q += 4;
             }
             if (q + 2 <= q1) {
                 const dv2 a0 = vj[q], u0 = vu[q], a1 = vj[q + 1], u1 = vu[q + 1];
