@@ -56,9 +56,11 @@ int run_t2map(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, const 
     HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
     rc = c->grids.ensure(c->err, o, c->stream);
     if (rc) return rc;
-    // the shared basis of SetFlipAngle, nTE x nT2 column-major (= the n x m transposed copy)
+    // the shared basis of SetFlipAngle, nTE x nT2 column-major (the column-major copy without its column padding)
     if (fixed && o->SaveNNLSBasis)
-        HIPCHECK(c, hipMemcpyAsync(d->decaybasis, c->grids.AfixedT.p, sizeof(double) * m * n, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHECK(c, hipMemcpy2DAsync(d->decaybasis, sizeof(double) * m, c->grids.AfixedT.p,
+                                     sizeof(double) * dmi::basis_ld(m), sizeof(double) * m, n, hipMemcpyDeviceToDevice,
+                                     c->stream));
     HIPCHECK(c, c->vox.ensure(sizeof(int64_t) * std::max<int64_t>(V, 1)));
     HIPCHECK(c, c->lb.counter.ensure(sizeof(unsigned) * 4));
     HIPCHECK(c, c->lb.counters.ensure(sizeof(unsigned long long) * 4));

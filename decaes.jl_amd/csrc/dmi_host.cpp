@@ -162,17 +162,22 @@ int Grids::ensure(std::string &err, const dmi_t2map_opts *o, hipStream_t s) {
     g.E2 = E2.as<double>();
     g.alphas = alphas.as<double>();
     const size_t bytes = sizeof(double) * (size_t)m * n * K;
+    // column-major copies: columns of stride basis_ld(m); the pad row of an odd m stays zero
+    const size_t bytesT = sizeof(double) * (size_t)dmi::basis_ld(m) * n * K;
     if (search) {
         DMI_HIP(err, As.ensure(bytes));
         DMI_HIP(err, dAs.ensure(bytes));
-        DMI_HIP(err, AsT.ensure(bytes));
-        DMI_HIP(err, dAsT.ensure(bytes));
+        DMI_HIP(err, AsT.ensure(bytesT));
+        DMI_HIP(err, dAsT.ensure(bytesT));
+        DMI_HIP(err, hipMemsetAsync(AsT.p, 0, bytesT, s));
+        DMI_HIP(err, hipMemsetAsync(dAsT.p, 0, bytesT, s));
         g.As = As.as<double>();
         g.dAs = dAs.as<double>();
         g.AsT = AsT.as<double>();
     } else {
         DMI_HIP(err, Afixed.ensure(bytes));
-        DMI_HIP(err, AfixedT.ensure(bytes));
+        DMI_HIP(err, AfixedT.ensure(bytesT));
+        DMI_HIP(err, hipMemsetAsync(AfixedT.p, 0, bytesT, s));
         g.As = Afixed.as<double>();
         g.dAs = nullptr;
         g.AsT = AfixedT.as<double>();
@@ -216,10 +221,9 @@ dmi::T2mapParams t2map_params(const dmi_t2map_opts *o, const Grids &g, const dou
     p.E2 = g.E2.as<double>();
     p.logT2 = g.logT2.as<double>();
     p.alphas = g.alphas.as<double>();
-    p.As = fixed ? nullptr : g.As.as<double>();
     p.AsT = fixed ? nullptr : g.AsT.as<double>();
     p.dAsT = fixed ? nullptr : g.dAsT.as<double>();
-    p.Afixed = fixed ? g.Afixed.as<double>() : nullptr;
+    p.Afixed = fixed ? g.AfixedT.as<double>() : nullptr;
     p.alpha_in = d_alpha;
     p.dist = d->dist; p.gdn = d->gdn; p.ggm = d->ggm; p.gva = d->gva; p.fnr = d->fnr; p.snr = d->snr; p.alpha = d->alpha;
     p.resnorm = o->SaveResidualNorm ? d->resnorm : nullptr;

@@ -13,6 +13,10 @@ constexpr int kMaxAngles = 64;    // one lane per flip-angle grid point
 constexpr int kTikhCache = 8;     // NNLSTikhonovRegProblemCache size (src/lsqnonneg.jl:396)
 constexpr int kLcurveCap = 128;   // L-curve cache capacity per voxel (reference starts at 64 and grows)
 
+// Column stride of the column-major bases in device memory (grid bases, fixed basis, the voxel's basis): m rounded
+// up to even, so every column starts 16-byte aligned and is copied into LDS with 16-byte global-to-LDS loads.
+__host__ __device__ inline int basis_ld(int m) { return (m + 1) & ~1; }
+
 enum StatusBits : int {
     kStatusLcurveOverflow = 1,
     kStatusNnlsMaxIter = 2,
@@ -30,7 +34,7 @@ struct GridParams {
     const double *alphas; // K
     double *As;           // K x m x n, row-major per angle
     double *dAs;          // K x m x n or nullptr
-    double *AsT;          // K x n x m (columns contiguous) or nullptr
+    double *AsT;          // K x n columns of stride basis_ld(m) (column-major per angle)
 };
 
 struct T2mapParams {
@@ -47,10 +51,9 @@ struct T2mapParams {
     const double *E2;     // n
     const double *logT2;  // n
     const double *alphas; // K
-    const double *As;     // K x m x n row-major grid bases
-    const double *AsT;    // K x n x m (columns contiguous)
-    const double *dAsT;   // d/dalpha, K x n x m
-    const double *Afixed; // m x n row-major (SetFlipAngle)
+    const double *AsT;    // K grid bases, n columns of stride basis_ld(m) each
+    const double *dAsT;   // d/dalpha, same layout
+    const double *Afixed; // the SetFlipAngle basis, n columns of stride basis_ld(m)
     const double *alpha_in;
     double *scratch;      // per-workgroup slots
     int64_t slot_stride;  // doubles per slot

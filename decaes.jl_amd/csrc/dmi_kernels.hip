@@ -415,7 +415,8 @@ __device__ __forceinline__ void epg_curve(bool general, double alpha, double bet
 }
 
 // Grid bases A_k and dA_k/dalpha for every flip angle of the search (EPGBasisSetEnsemble with
-// ∇epg_decay_basis!, src/T2mapSEcorr.jl:339-371). Block = (angle k, T2 pair); row-major K x m x n.
+// ∇epg_decay_basis!, src/T2mapSEcorr.jl:339-371). Block = (angle k, T2 pair); row-major K x m x n, and the
+// column-major copy AsT with column stride basis_ld(m) that the solves and the flip-angle gradient read.
 template <int LPC>
 __global__ __launch_bounds__(64) void grid_basis_kernel(GridParams p) {
     constexpr int per = kWave / LPC;
@@ -424,7 +425,8 @@ __global__ __launch_bounds__(64) void grid_basis_kernel(GridParams p) {
     const bool valid = j < p.n;
     const int jj = valid ? j : p.n - 1;
     const int64_t base = (int64_t)k * p.m * p.n + jj;
-    gdbl *outT = p.AsT ? (gdbl *)p.AsT + (int64_t)k * p.m * p.n + (int64_t)jj * p.m : nullptr;
+    const int ldg = basis_ld(p.m);
+    gdbl *outT = p.AsT ? (gdbl *)p.AsT + ((int64_t)k * p.n + jj) * ldg : nullptr;
     if (p.dAs && p.fp32)
         epg_curve<LPC, DualF>(p.general, p.alphas[k], p.beta, p.E1, p.E2[jj], p.m, s, valid, (gdbl *)p.As + base,
                               (gdbl *)p.dAs + base, p.n, outT);
@@ -439,13 +441,15 @@ __global__ __launch_bounds__(64) void grid_basis_kernel(GridParams p) {
                                nullptr, p.n, outT);
 }
 
-// dA_k/dalpha with columns contiguous (K x n x m), for the coalesced row-lane gradient of the search.
+// dA_k/dalpha with columns contiguous (K x n columns of stride basis_ld(m)), for the coalesced row-lane gradient of
+// the search.
 __global__ void transpose_kernel(const double *src, double *dst, int m, int n, int K) {
     const int64_t total = (int64_t)K * m * n;
+    const int ldg = basis_ld(m);
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
         int64_t k = t / ((int64_t)m * n), r = t % ((int64_t)m * n);
         int j = (int)(r / m), i = (int)(r % m);
-        dst[t] = src[(k * m + i) * n + j];
+        dst[(k * n + j) * ldg + i] = src[(k * m + i) * n + j];
     }
 }
 
@@ -524,7 +528,7 @@ struct Wave {
     double wl;
     // per-workgroup global scratch
     gdbl *A0s, *cx, *lc;
-    cgdbl *A0;   // basis used by the regularization solves (A0s or the fixed basis), row-major
+    cgdbl *A0;   // basis used by the regularization solves (A0s or the fixed basis), column-major, stride basis_ld(m)
     int ccur;
     unsigned long long flops, flops32, solves;
     int status;
@@ -965,35 +969,68 @@ struct Wave {
         if (tikh) DMI_TIKH_ROWS(__popcll(dmask));
     }
 
+    // Rows [0, basis_ld(m)) of the n columns of a column-major global basis (column stride basis_ld(m), 16-byte
+    // aligned) straight into the LDS matrix: one 16-byte global-to-LDS load per lane and column from lanes
+    // [0, basis_ld(m) / 2), which lands lane l's bytes at the column start + 16 l. No register holds the data, so
+    // all n loads stay in flight together whatever the caller keeps live; basis_wait() completes them.
+    __device__ __forceinline__ void load_basis(cgdbl *__restrict__ src) {
+        const int ldg = basis_ld(m);
+        if (L < ldg / 2)
+            for (int j = 0; j < n; ++j)
+                __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void *)(src + (size_t)j * ldg + 2 * L),
+                                                 (__attribute__((address_space(3))) void *)(A + j * lda), 16, 0, 0);
+    }
+    // Callers follow it with __syncthreads(), whose LDS fence keeps every read of the copied rows after the wait.
+    __device__ __forceinline__ static void basis_wait() { asm volatile("s_waitcnt vmcnt(0)"); }
+
     // NNLSProblem solve! with the reference warm start (src/lsqnonneg.jl:30-84), or the Tikhonov form
-    // (:86-164) when tikh. A0g: global row-major m x n basis; right-hand side b0 in LDS.
+    // (:86-164) when tikh. A0g: global column-major m x n basis (column stride basis_ld(m)); right-hand side b0
+    // in LDS.
     __device__ __forceinline__ void solve(cgdbl *__restrict__ A0g, bool tikh, double mu, int &nsetp, double &rnorm) {
         DMI_PT0();
         const int M = tikh ? m + n : m;
+        const int ldg = basis_ld(m);
+        // rows L and L + 64 of the last column (m <= 124) in registers first, so that no global read waits behind
+        // the basis copy, which then runs while the last column is fitted
+        cgdbl *an = A0g + (size_t)(n - 1) * ldg;
+        const double a0 = (L < m) ? an[L] : 0.0, a1 = (L + kWave < m) ? an[L + kWave] : 0.0;
         double pd = 0.0;
-        for (int i = L; i < m; i += kWave) { double a = A0g[i * n + n - 1]; pd += a * a; }
+        if (L < m) pd += a0 * a0;
+        if (L + kWave < m) pd += a1 * a1;
+        // pd (hence both loads) is formed before the copy is issued: otherwise its wait would cover the copy too
+        asm volatile("" ::"v"(pd));
+        load_basis(A0g);
         double den = wsum(pd);
         if (tikh) den += mu * mu;
         double px = 0.0;
-        for (int i = L; i < m; i += kWave) px += (A0g[i * n + n - 1] / den) * b0[i];
+        if (L < m) px += (a0 / den) * b0[L];
+        if (L + kWave < m) px += (a1 / den) * b0[L + kWave];
         double xj = wsum(px);
-        for (int i = L; i < m; i += kWave) zz[i] = b0[i] - A0g[i * n + n - 1] * xj;
+        if (L < m) zz[L] = b0[L] - a0 * xj;
+        if (L + kWave < m) zz[L + kWave] = b0[L + kWave] - a1 * xj;
+        // Tikhonov rows below the copied ones are cleared while the copy is in flight; a pad row (odd m) after it
+        if (tikh && L < n)
+            for (int i = ldg; i < M; ++i) Aat(i, L) = 0.0;
+        basis_wait();
+        if (tikh && L < n && ldg > m) Aat(m, L) = 0.0;
         __syncthreads();
         bool pos = false;
         if (L < n) {
+            // w_j = sum_i A_ij zz_i over the rows in order, two rows per 16-byte load
+            const ldbl *cj = A + L * lda;
             double wj = 0.0;
-#pragma unroll 16
-            for (int i = 0; i < m; ++i) {
-                double a = A0g[i * n + L];
-                wj += a * zz[i];
-                Aat(i, L) = a;
+            int i = 0;
+#pragma unroll 4
+            for (; i + 2 <= m; i += 2) {
+                const dv2 a = *(cldv2 *)(cj + i), z = *(cldv2 *)(zz + i);
+                wj += a.x * z.x;
+                wj += a.y * z.y;
             }
+            if (i < m) wj += cj[i] * zz[i];
             if (L < n - 1) { wl = wj; pos = !(wj <= 0); }
             x[L] = 0.0;
             idx[L] = L;
             diag[L] = 0;
-            if (tikh)
-                for (int i = m; i < M; ++i) Aat(i, L) = 0.0;
         }
         unsigned long long bal = ballot(pos);
         if (L == n - 1) wl = (bal == 0) ? 1.0 : 0.0;
@@ -1272,8 +1309,8 @@ __device__ __forceinline__ void reg_mdp(Wave &W, double delta, double &mu_out, d
 __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
     const int m = W.m, n = W.n, L = W.L, lda = W.lda;
     DMI_PT0();
-    if (L < n)
-        for (int i = 0; i < m; ++i) W.Aat(i, L) = W.A0[(int64_t)i * n + L];
+    W.load_basis(W.A0);
+    W.basis_wait();
     __syncthreads();
     N = min(m, n);
     // N x N Gram matrix at rows [m, m + N) of columns [0, N) (lda >= m + n leaves the room), built with FP64
@@ -1345,11 +1382,16 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
         if (L > k && L < N) {
             double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
             int c = k + 1;
+            // the step's loads are issued together (the scheduler would otherwise trade them for registers)
             for (; c + 4 <= N; c += 4) {
-                p0 = fma(G(L, c), vv[c], p0);
-                p1 = fma(G(L, c + 1), vv[c + 1], p1);
-                p2 = fma(G(L, c + 2), vv[c + 2], p2);
-                p3 = fma(G(L, c + 3), vv[c + 3], p3);
+                double g[4], v[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) { g[r] = G(L, c + r); v[r] = vv[c + r]; }
+                __builtin_amdgcn_sched_barrier(0);
+                p0 = fma(g[0], v[0], p0);
+                p1 = fma(g[1], v[1], p1);
+                p2 = fma(g[2], v[2], p2);
+                p3 = fma(g[3], v[3], p3);
             }
             for (; c < N; ++c) p0 = fma(G(L, c), vv[c], p0);
             pr = (p0 + p1) + (p2 + p3);
@@ -1366,6 +1408,7 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
                 double g[4], q[4], v[4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) { g[r] = G(L, c + r); q[r] = qv[c + r]; v[r] = vv[c + r]; }
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) G(L, c + r) = g[r] - 2.0 * (vr * q[r] + qr * v[r]);
             }
@@ -1845,29 +1888,40 @@ __device__ __forceinline__ void hermite_minimize(double a, double b, double u0, 
 
 struct FlipSearch {
     Wave &W;
-    cgdbl *As, *AsT, *dAsT;
+    cgdbl *AsT, *dAsT;  // K column-major bases and their alpha-derivatives, column stride basis_ld(m)
     int K, numeval, mineval;
     unsigned long long seen;
     double al, u, du;  // lane k: grid angle, loss, gradient
 
     __device__ __forceinline__ void eval(int k) {
         const int m = W.m, n = W.n;
-        cgdbl *A = As + (size_t)k * m * n;
-        cgdbl *AT = AsT + (size_t)k * m * n, *dAT = dAsT + (size_t)k * m * n;
+        const int ldg = basis_ld(m);
+        cgdbl *AT = AsT + (size_t)k * n * ldg, *dAT = dAsT + (size_t)k * n * ldg;
         int ns;
         double rn;
-        W.solve(A, false, 0.0, ns, rn);
+        W.solve(AT, false, 0.0, ns, rn);
         double ell = rn * rn;
         double part = 0.0;
         for (int i = W.L; i < m; i += kWave) {
             double ax = 0.0, dax = 0.0;
-            // every column, without a branch on x_j > 0, so the loads of all columns are issued together: an
-            // inactive column has x_j = 0 exactly and fma(0, a, s) = s for finite a
-#pragma unroll 8
-            for (int j = 0; j < n; ++j) {
-                const double xj = W.x[j];
-                ax = fma(xj, AT[j * m + i], ax);
-                dax = fma(xj, dAT[j * m + i], dax);
+            // every column, without a branch on x_j > 0, eight columns per step with all sixteen loads issued
+            // together: an inactive column has x_j = 0 exactly and fma(0, a, s) = s for finite a, which also covers
+            // the columns past n - 1 that a last partial step re-reads
+            for (int j0 = 0; j0 < n; j0 += 8) {
+                double a[8], da[8], xv[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int j = min(j0 + k, n - 1);
+                    a[k] = AT[j * ldg + i];
+                    da[k] = dAT[j * ldg + i];
+                    xv[k] = (j0 + k < n) ? W.x[j] : 0.0;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    ax = fma(xv[k], a[k], ax);
+                    dax = fma(xv[k], da[k], dax);
+                }
             }
             ax -= W.b0[i];
             part += dax * ax;
@@ -1977,6 +2031,7 @@ template <typename T>
 __device__ __forceinline__ void voxel_basis_t(Wave &W, bool general, double alpha, double beta, double E1, cgdbl *E2,
                                               gdbl *A0out) {
     const int m = W.m, n = W.n, L = W.L;
+    const int ldg = basis_ld(m);  // column-major output: column j at A0out + j * ldg
     if constexpr (std::is_same<T, double>::value) {
         if (!general) {
             // two curves per lane group and pass (epg_const_pair)
@@ -1985,12 +2040,13 @@ __device__ __forceinline__ void voxel_basis_t(Wave &W, bool general, double alph
                 for (int jp = 0; jp < n; jp += 4) {
                     const int j1 = jp + h, j2 = jp + 2 + h;
                     const int jj1 = j1 < n ? j1 : n - 1, jj2 = j2 < n ? j2 : n - 1;
-                    epg_const_pair<32>(alpha, E1, E2[jj1], E2[jj2], m, s, j1 < n, j2 < n, A0out + jj1, A0out + jj2, n);
+                    epg_const_pair<32>(alpha, E1, E2[jj1], E2[jj2], m, s, j1 < n, j2 < n, A0out + jj1 * ldg,
+                                       A0out + jj2 * ldg, 1);
                 }
             } else {
                 for (int j = 0; j < n; j += 2) {
                     const int j2 = j + 1 < n ? j + 1 : n - 1;
-                    epg_const_pair<64>(alpha, E1, E2[j], E2[j2], m, L, true, j + 1 < n, A0out + j, A0out + j2, n);
+                    epg_const_pair<64>(alpha, E1, E2[j], E2[j2], m, L, true, j + 1 < n, A0out + j * ldg, A0out + j2 * ldg, 1);
                 }
             }
             return;
@@ -2002,11 +2058,11 @@ __device__ __forceinline__ void voxel_basis_t(Wave &W, bool general, double alph
             int j = jp + h;
             bool valid = j < n;
             int jj = valid ? j : n - 1;
-            epg_curve<32, T>(general, alpha, beta, E1, E2[jj], m, s, valid, A0out + jj, nullptr, n, nullptr);
+            epg_curve<32, T>(general, alpha, beta, E1, E2[jj], m, s, valid, A0out + jj * ldg, nullptr, 1, nullptr);
         }
     } else {
         for (int j = 0; j < n; ++j)
-            epg_curve<64, T>(general, alpha, beta, E1, E2[j], m, L, true, A0out + j, nullptr, n, nullptr);
+            epg_curve<64, T>(general, alpha, beta, E1, E2[j], m, L, true, A0out + j * ldg, nullptr, 1, nullptr);
     }
 }
 __device__ __forceinline__ void voxel_basis(Wave &W, bool fp32, bool general, double alpha, double beta, double E1,
@@ -2024,7 +2080,7 @@ __device__ __forceinline__ void voxel_basis(Wave &W, bool fp32, bool general, do
 
 // ------------------------------------ the voxel kernel ------------------------------------
 template <int REG>
-__global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void t2map_kernel(T2mapParams p) {
     extern __shared__ double smem[];
     Wave W;
     W.L = lane_id();
@@ -2049,10 +2105,10 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
     W.diag = si; si += ne;
     W.stk = si;  // 3 (K + 2) ints: the depth-limited bisection stack holds at most nRefAnglesMin + 1 <= K + 1 boxes
     gdbl *slot = (gdbl *)p.scratch + (int64_t)blockIdx.x * p.slot_stride;
-    // slot: the voxel's row-major basis (one copy, so the slots of the resident workgroups stay within L2), the
+    // slot: the voxel's column-major basis (one copy, so the slots of the resident workgroups stay within L2), the
     // Tikhonov cache solutions and the L-curve cache
     W.A0s = slot;
-    W.cx = slot + (int64_t)p.m * p.n;
+    W.cx = slot + (int64_t)basis_ld(p.m) * p.n;
     W.lc = W.cx + (int64_t)kTikhCache * p.n;
     W.flops = 0;
     W.flops32 = 0;
@@ -2088,7 +2144,7 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
             if (p.mode == kModeB1) {
                 alpha_v = p.alpha_in[I];
             } else {
-                FlipSearch fs{W, (cgdbl *)p.As, (cgdbl *)p.AsT, (cgdbl *)p.dAsT, p.K, 0, p.mineval, 0ull, 0.0, 0.0, 0.0};
+                FlipSearch fs{W, (cgdbl *)p.AsT, (cgdbl *)p.dAsT, p.K, 0, p.mineval, 0ull, 0.0, 0.0, 0.0};
                 fs.al = (L < p.K) ? p.alphas[L] : p.alphas[p.K - 1];
                 fs.u = NAN;
                 fs.du = NAN;
@@ -2123,7 +2179,7 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
         double pr2 = 0.0, prs = 0.0;
         for (int i = L; i < m; i += kWave) {
             double fit = 0.0;
-            for (int j = 0; j < n; ++j) fit += W.A0[i * n + j] * W.xf[j];
+            for (int j = 0; j < n; ++j) fit += W.A0[j * basis_ld(m) + i] * W.xf[j];
             double bi = W.b0[i] * max_signal;
             double res = fit - bi;
             if (i == L) res0 = res; else res1 = res;
@@ -2156,7 +2212,7 @@ __global__ __launch_bounds__(64) void t2map_kernel(T2mapParams p) {
         }
         if (L < n) p.dist[I + V * L] = xs;
         if (p.decaybasis && p.mode != kModeFixed && L < n)
-            for (int i = 0; i < m; ++i) p.decaybasis[I + V * (i + (int64_t)m * L)] = W.A0[i * n + L];
+            for (int i = 0; i < m; ++i) p.decaybasis[I + V * (i + (int64_t)m * L)] = W.A0[L * basis_ld(m) + i];
         __syncthreads();
         DMI_PT1(kPhVoxel);
     }
@@ -2230,7 +2286,7 @@ size_t t2map_lds_bytes(int m, int n, int lda, int K) {
 }
 
 int64_t t2map_slot_doubles(int m, int n) {
-    int64_t s = (int64_t)m * n + (int64_t)kTikhCache * n + 19 * (int64_t)kLcurveCap;
+    int64_t s = (int64_t)basis_ld(m) * n + (int64_t)kTikhCache * n + 19 * (int64_t)kLcurveCap;
     return (s + 31) / 32 * 32;
 }
 

@@ -234,8 +234,9 @@ int bind_and_select(dmi_server *S, StreamState &st, Request *r) {
     if (rc) return rc;
     const bool fixed = !std::isnan(r->o.SetFlipAngle);
     if (fixed && r->o.SaveNNLSBasis)
-        DMI_HIP(e, hipMemcpyAsync(r->d_out[11], st.grids.AfixedT.p, sizeof(double) * r->o.nTE * r->o.nT2,
-                                  hipMemcpyDeviceToDevice, st.s));
+        DMI_HIP(e, hipMemcpy2DAsync(r->d_out[11], sizeof(double) * r->o.nTE, st.grids.AfixedT.p,
+                                    sizeof(double) * dmi::basis_ld(r->o.nTE), sizeof(double) * r->o.nTE, r->o.nT2,
+                                    hipMemcpyDeviceToDevice, st.s));
     // scratch for the largest grid any regularization method can launch with these sizes, so it never grows
     // while a chunk runs
     const int m = r->o.nTE, n = r->o.nT2;
