@@ -668,21 +668,25 @@ struct Wave {
                 cj[lo] = a;
             }
             q = q0;
-            // four row pairs per step: all loads are issued before the stores of the step
-            for (; q + 4 <= q1; q += 4) {
-                dv2 a[4], u[4], bb[4];
+            // NB row pairs per step: all loads are issued before the stores of the step; eight, then four, then
+            // single pairs (w0 and w1 take the rows in order whatever the step)
+            auto update = [&](auto nb, int qs) {
+                constexpr int NB = decltype(nb)::value;
+                dv2 a[NB], u[NB], bb[NB];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) { a[k] = vj[q + k]; u[k] = vu[q + k]; bb[k] = vb[q + k]; }
+                for (int k = 0; k < NB; ++k) { a[k] = vj[qs + k]; u[k] = vu[qs + k]; bb[k] = vb[qs + k]; }
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
+                for (int k = 0; k < NB; ++k) {
                     a[k].x = fma(sm, u[k].x, a[k].x);
                     a[k].y = fma(sm, u[k].y, a[k].y);
                     w0 = fma(a[k].x, bb[k].x, w0);
                     w1 = fma(a[k].y, bb[k].y, w1);
                 }
 #pragma unroll
-                for (int k = 0; k < 4; ++k) vj[q + k] = a[k];
-            }
+                for (int k = 0; k < NB; ++k) vj[qs + k] = a[k];
+            };
+            for (; q + 8 <= q1; q += 8) update(std::integral_constant<int, 8>(), q);
+            if (q + 4 <= q1) { update(std::integral_constant<int, 4>(), q); q += 4; }
             for (; q < q1; ++q) {
                 dv2 a = vj[q];
                 const dv2 u = vu[q], bb = vb[q];
