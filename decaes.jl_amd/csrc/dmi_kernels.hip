@@ -125,6 +125,12 @@ __device__ __forceinline__ void wargmin(double &v, int &i) {
     i = hit ? __ffsll((long long)hit) - 1 : INT_MAX;
 }
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
+// Orders the wave's LDS accesses where a workgroup barrier would: every workgroup here is one wave, whose LDS
+// operations complete in issue order, so no wait is needed (a barrier compiles to lgkmcnt(0), a stall until every
+// outstanding LDS operation has completed); the wavefront-scope fence only keeps the compiler from moving LDS
+// accesses across it. Not for global memory written by one lane and read by another.
+__device__ __forceinline__ void lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
 __device__ __forceinline__ int first_set(unsigned long long m) { return m ? __ffsll((long long)m) - 1 : -1; }
 __device__ __forceinline__ int last_set(unsigned long long m) { return m ? 63 - __clzll((long long)m) : -1; }
 
@@ -607,7 +613,7 @@ struct Wave {
                     if (i < mr) { ci[i] = c[k]; cj[i] = t[k]; }
                 }
         }
-        __syncthreads();
+        lds_order();
         return tau;
     }
 
@@ -704,7 +710,7 @@ struct Wave {
             wl = w0 + w1;
         }
         flops += 6ull * (m1 - j1) * (n - j1 - 1);
-        __syncthreads();
+        lds_order();
     }
 
     // compute_dual! (:441-470): w_j = sum_{i >= j1} A_ij b_i for j >= j1, rows [j1, m1).
@@ -743,7 +749,7 @@ struct Wave {
             wl = 0.0;
         }
         flops += 2ull * (m1 - j1) * (n - j1);
-        __syncthreads();
+        lds_order();
     }
 
     // solve_triangular_system! (:506-524) on R = A[0:k, 0:k]; lane i holds z_i in a register. The reference's
@@ -802,12 +808,12 @@ struct Wave {
         for (;;) {
             if (L == imv) xp = 0.0;
             for (int r = imv + 1 + L; r < mrows; r += kWave) Aat(r, imv) = 0.0;
-            __syncthreads();
+            lds_order();
             if (imv != nsetp - 1) {
                 for (int i = imv + 1; i <= nsetp - 1; ++i) {
                     double aa = rfl(Aat(i - 1, i)), bb = rfl(Aat(i, i));
                     double sig = hypot(aa, bb), cc = aa / sig, ss = bb / sig;
-                    __syncthreads();
+                    lds_order();
                     const int j = L;
                     if (j < n && (j >= i || j == imv)) {
                         if (j == i) { Aat(i - 1, i) = sig; Aat(i, i) = 0.0; }
@@ -823,7 +829,7 @@ struct Wave {
                         b[i] = fma(-ss, u, cc * v);
                     }
                     flops += 6ull * (n + 1);
-                    __syncthreads();
+                    lds_order();
                 }
                 for (int r = L; r < mrows; r += kWave) {
                     double t = Aat(r, imv);
@@ -837,7 +843,7 @@ struct Wave {
                 const double xdn = __shfl_down(xp, 1);
                 if (L >= imv && L < nsetp - 1) { pidx = pdn; xp = xdn; }
                 else if (L == nsetp - 1) { pidx = pim; xp = xim; }
-                __syncthreads();
+                lds_order();
             }
             nsetp -= 1;
             bool bad = (L < nsetp) && (xp <= 0);
@@ -878,7 +884,7 @@ struct Wave {
                 const bool newdiag = tikh && !((dmask >> cmax) & 1ull);
                 if (newdiag) {
                     if (L == 0) Aat(mr, jmax) = lam;
-                    __syncthreads();
+                    lds_order();
                 }
                 const int mrh = tikh ? min(mr + 1, M) : mr;
                 {
@@ -889,7 +895,7 @@ struct Wave {
                 if (tau >= 0) break;
                 if (L == jmax) { wv = 0.0; wl = 0.0; }
                 if (L == 0 && tikh && mr < M) Aat(mr, jmax) = 0.0;
-                __syncthreads();
+                lds_order();
             }
             DMI_LAP(kPhSelect);
             if (term) break;
@@ -964,7 +970,7 @@ struct Wave {
             idx[L] = pidx;
             diag[L] = (int)((dmask >> L) & 1ull);
         }
-        __syncthreads();
+        lds_order();
         const int Mlim = tikh ? M : mr;
         double part = 0.0;
         for (int i = nsetp + L; i < Mlim; i += kWave) { double bi = b[i]; part = fma(bi, bi, part); }
