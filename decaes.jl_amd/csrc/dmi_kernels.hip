@@ -1362,7 +1362,7 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
             }
     }
     W.flops += 2ull * N * N * (m >= n ? m : n);
-    __syncthreads();
+    lds_order();
     DMI_LAP(kPhGram);
     ldbl *dv = W.b, *ev = W.b + N, *vv = W.zz, *qv = W.zz + N;
     for (int k = 0; k < N - 2; ++k) {
@@ -1371,7 +1371,7 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
         double x1 = rfl(G(k + 1, k));
         if (nrm == 0.0) {
             if (L == 0) ev[k] = x1;
-            __syncthreads();
+            lds_order();
             continue;
         }
         double ah = -copysign(nrm, x1);
@@ -1380,13 +1380,13 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
         double vn = sqrt(wsum(vr * vr));
         if (vn == 0.0) {
             if (L == 0) ev[k] = x1;
-            __syncthreads();
+            lds_order();
             continue;
         }
         vr /= vn;
         if (L < N) vv[L] = vr;
         if (L == 0) ev[k] = ah;
-        __syncthreads();
+        lds_order();
         // p = G v over the trailing rows: four partial sums, so the FMA chain is a quarter as long
         double pr = 0.0;
         if (L > k && L < N) {
@@ -1409,7 +1409,7 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
         double Kq = wsum(vr * pr);
         double qr = pr - Kq * vr;
         if (L < N) qv[L] = qr;
-        __syncthreads();
+        lds_order();
         // rank-2 update, four columns per step with every load of the step ahead of its stores (the loads of
         // qv / vv could alias the stores to G as far as the compiler knows, so a plain loop would serialize)
         if (L > k && L < N) {
@@ -1425,11 +1425,11 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
             for (; c < N; ++c) G(L, c) -= 2.0 * (vr * qv[c] + qr * vv[c]);
         }
         W.flops += 8ull * (N - k) * (N - k);
-        __syncthreads();
+        lds_order();
     }
     if (L < N) dv[L] = G(L, L);
     if (N >= 2 && L == 0) ev[N - 2] = G(N - 1, N - 2);
-    __syncthreads();
+    lds_order();
     DMI_LAP(kPhTridiag);
     // Gershgorin interval
     double lo = INFINITY, hi = -INFINITY;
@@ -1994,11 +1994,11 @@ struct FlipSearch {
             int sp = 0;
             if (W.L == 0) { W.stk[0] = 0; W.stk[1] = K - 1; W.stk[2] = depth; }
             sp = 1;
-            __syncthreads();
+            lds_order();
             while (sp > 0) {
                 sp -= 1;
                 int lo = rfl(W.stk[3 * sp]), hi = rfl(W.stk[3 * sp + 1]), dp = rfl(W.stk[3 * sp + 2]);
-                __syncthreads();
+                lds_order();
                 if (dp <= 0) continue;
                 evaluate_box(lo, hi, false, 0.0);
                 if (numeval >= mineval) continue;
@@ -2008,7 +2008,7 @@ struct FlipSearch {
                     W.stk[3 * sp + 3] = lo; W.stk[3 * sp + 4] = mid; W.stk[3 * sp + 5] = dp - 1;
                 }
                 sp += 2;
-                __syncthreads();
+                lds_order();
             }
             if (numeval >= mineval) break;
         }
@@ -2142,10 +2142,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
             W.b0[i] = bi;
         }
         double max_signal = wmaxd(mx);
-        __syncthreads();
+        lds_order();
         if (max_signal > 0)
             for (int i = L; i < m; i += kWave) W.b0[i] /= max_signal;
-        __syncthreads();
+        lds_order();
         double alpha_v;
         if (p.mode == kModeFixed) {
             alpha_v = p.setflip;
