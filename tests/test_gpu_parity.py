@@ -231,6 +231,43 @@ def test_size_limits(dmi, oracle, shape, m, n, K, reg):
         assert np.mean(rel <= 1e-6) >= 0.9
 
 
+@pytest.mark.parametrize("m,reg,setflip", [(33, "gcv", None), (25, "chi2", 165.0)])
+def test_odd_echo_count_basis_layout(dmi, oracle, m, reg, setflip):
+    """An odd echo count pads every column of the device bases to an even stride (grid bases, the fixed basis and
+    the voxel's basis). Distributions and angles match the oracle; the saved per-voxel basis equals the oracle's
+    fixed basis (SetFlipAngle) and reproduces the saved decay curve from the distribution (both paths)."""
+    print(f"[parity] case=odd_m m={m} {reg} setflip={setflip}", flush=True)
+    shape, TE, n, K, T2R = (4, 3, 2), 10e-3, 40, 32, (10e-3, 2.0)
+    img, _ = dmi.synth_mse(shape, m, TE, mask_fraction=1.0, seed=23)
+    extra = dict(Chi2Factor=1.02) if reg == "chi2" else {}
+    if setflip is not None:
+        extra["SetFlipAngle"] = setflip
+    opts = dmi.T2mapOptions.from_image(img, TE=TE, nT2=n, T2Range=T2R, Reg=reg, nRefAngles=K, nRefAnglesMin=5,
+                                       Silent=True, SaveRegParam=True, SaveDecayCurve=True, SaveNNLSBasis=True,
+                                       **extra)
+    maps, dist = dmi.T2mapSEcorr(img, opts)
+    oo = oracle.make_t2map_opts(shape, m, TE, n, T2R, reg, nRefAngles=K, nRefAnglesMin=5,
+                                Chi2Factor=extra.get("Chi2Factor"), SetFlipAngle=setflip)
+    omaps, odist, nv = oracle.t2map(img, oo, nthreads=os.cpu_count())
+    assert np.array_equal(np.isnan(dist), np.isnan(odist))
+    mask, rel, da = _errors(maps, dist, omaps, odist, T2R)
+    print(f"[parity] odd_m: {mask.sum()} voxels, max dist rel {rel.max():.2e}, max alpha {da.max():.2e}", flush=True)
+    assert mask.sum() == np.prod(shape)
+    assert np.max(da) <= 1e-2
+    assert np.mean(rel <= 1e-6) >= 0.9
+    if setflip is not None:
+        # one shared basis (m, n), copied out of the padded device layout
+        assert maps["decaybasis"].shape == (m, n)
+        assert np.max(np.abs(maps["decaybasis"] - omaps["decaybasisset"])) <= 1e-13
+        basis = np.broadcast_to(maps["decaybasis"], (int(mask.sum()), m, n))
+    else:
+        basis = maps["decaybasis"][mask]                  # (voxels, m, n)
+        assert basis.shape[1:] == (m, n)
+    fit = np.einsum("vin,vn->vi", basis, dist[mask])
+    curve = maps["decaycurve"][mask]
+    assert np.max(np.abs(fit - curve)) <= 1e-12 * np.max(np.abs(curve))
+
+
 def test_float32_inputs(dmi):
     """Float32 arrays: outputs are the FP64 path's on the widened input, rounded once to Float32."""
     print("[parity] case=float32", flush=True)
