@@ -2,8 +2,8 @@
 //
 // Execution model: one 64-lane wavefront per voxel, one wave per workgroup, a persistent grid pulling
 // voxel indices from a device counter. Per-voxel dense state (the Lawson-Hanson workspace, right-hand
-// side, duals, Tikhonov cache keys) lives in LDS; the per-voxel EPG basis, cached Tikhonov solutions and
-// the L-curve caches live in a per-workgroup global scratch slot that stays resident in L2. Lanes take the
+// side, duals, the Tikhonov cache with its solutions) lives in LDS; the per-voxel EPG basis and the L-curve
+// caches live in a per-workgroup global scratch slot that stays resident in L2. Lanes take the
 // natural parallel dimension of each step: columns for Householder applications and duals, rows for
 // reflector construction, residuals and column shifts, phase states for the EPG recursion, grid angles for
 // the flip-angle surrogate. Scalar control flow (active-set loop, Brent, bisection search, L-curve) is
@@ -527,13 +527,13 @@ struct Wave {
     // LDS. A is column-major with column stride lda (even, lda = 2 mod 4, lda >= mw rounded up to even):
     // a lane that owns a column streams it with 16-byte loads, and 16-byte loads of 16 consecutive lanes
     // at stride lda hit distinct banks. Every LDS double array starts 16-byte aligned.
-    ldbl *A, *b, *zz, *x, *xf, *b0, *cmu, *crn, *csemi;
+    ldbl *A, *b, *zz, *x, *xf, *b0, *cmu, *crn, *csemi, *cx;
     lint *idx, *diag, *stk;
     int lda, mw, m, n, L;
     // the dual w_j of position j = L (the reference's w), in a register: only this lane writes or reads it
     double wl;
     // per-workgroup global scratch
-    gdbl *A0s, *cx, *lc;
+    gdbl *A0s, *lc;
     cgdbl *A0;   // basis used by the regularization solves (A0s or the fixed basis), column-major, stride basis_ld(m)
     int ccur;
     unsigned long long flops, flops32, solves;
@@ -1060,7 +1060,7 @@ struct Wave {
 
     // ------------------------- Tikhonov cache (src/lsqnonneg.jl:392-444) -------------------------
     // A hit needs exactly equal mu; re-solving the same mu reproduces the same bits, so the cache is a pure
-    // memo. Entries keep mu, rnorm, ||x||^2 and x (global scratch).
+    // memo. Entries keep mu, rnorm, ||x||^2 and x (all in LDS).
     __device__ __forceinline__ void cache_reset() {
         if (L < kTikhCache) cmu[L] = NAN;
         __syncthreads();
@@ -2110,16 +2110,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     W.cmu = s; s += kTikhCache;
     W.crn = s; s += kTikhCache;
     W.csemi = s; s += kTikhCache;
+    W.cx = s; s += (size_t)kTikhCache * p.n;  // the cached Tikhonov solutions, n per entry
     lint *si = (lint *)s;
     W.idx = si; si += ne;
     W.diag = si; si += ne;
     W.stk = si;  // 3 (K + 2) ints: the depth-limited bisection stack holds at most nRefAnglesMin + 1 <= K + 1 boxes
     gdbl *slot = (gdbl *)p.scratch + (int64_t)blockIdx.x * p.slot_stride;
-    // slot: the voxel's column-major basis (one copy, so the slots of the resident workgroups stay within L2), the
-    // Tikhonov cache solutions and the L-curve cache
+    // slot: the voxel's column-major basis (one copy, so the slots of the resident workgroups stay within L2) and the
+    // L-curve cache
     W.A0s = slot;
-    W.cx = slot + (int64_t)basis_ld(p.m) * p.n;
-    W.lc = W.cx + (int64_t)kTikhCache * p.n;
+    W.lc = slot + (int64_t)basis_ld(p.m) * p.n;
     W.flops = 0;
     W.flops32 = 0;
     W.solves = 0;
@@ -2290,13 +2290,14 @@ int t2map_lda(int m, int n) {
 
 size_t t2map_lds_bytes(int m, int n, int lda, int K) {
     const size_t ne = (size_t)((n + 1) & ~1);
-    size_t d = (size_t)n * lda + 2 * (size_t)lda + 2 * ne + (size_t)((m + 1) & ~1) + 3 * kTikhCache;
+    size_t d = (size_t)n * lda + 2 * (size_t)lda + 2 * ne + (size_t)((m + 1) & ~1) + 3 * kTikhCache +
+               (size_t)kTikhCache * n;
     size_t i = 2 * ne + 3 * ((size_t)K + 2);
     return d * sizeof(double) + i * sizeof(int);
 }
 
 int64_t t2map_slot_doubles(int m, int n) {
-    int64_t s = (int64_t)basis_ld(m) * n + (int64_t)kTikhCache * n + 19 * (int64_t)kLcurveCap;
+    int64_t s = (int64_t)basis_ld(m) * n + 19 * (int64_t)kLcurveCap;
     return (s + 31) / 32 * 32;
 }
 
