@@ -17,6 +17,7 @@
 #include <complex.h>
 #undef I
 #define IU _Complex_I
+#include <float.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdatomic.h>
@@ -1106,6 +1107,154 @@ void orc_svdvals(const double *A, int m, int n, double *s) {
     free(U);
 }
 
+/* Singular values squared through the Gram matrix: eigenvalues of G = AᵀA (or AAᵀ when m < n) by cyclic
+ * Jacobi rotations, s = sqrt(max(λ, 0)). Each eigenvalue carries an absolute error of order eps·‖A‖², so a
+ * singular value far below ‖A‖ is lost. This is NOT what the reference does; it exists so tests can show what
+ * that loss does to the GCV regularization parameter (orc_set_svd_route(1)). */
+static void gram_svdvals(const double *A, int m, int n, double *s) {
+    int tr = m < n;
+    int c = tr ? m : n, r = tr ? n : m;
+    double *G = malloc(sizeof(double) * (size_t)c * c);
+    for (int p = 0; p < c; ++p)
+        for (int q = 0; q < c; ++q) {
+            double acc = 0;
+            for (int i = 0; i < r; ++i) {
+                double x = tr ? A[p + (size_t)i * m] : A[i + (size_t)p * m];
+                double y = tr ? A[q + (size_t)i * m] : A[i + (size_t)q * m];
+                acc += x * y;
+            }
+            G[p + (size_t)q * c] = acc;
+        }
+    for (int sweep = 0; sweep < 80; ++sweep) {
+        double off = 0, dn = 0;
+        for (int p = 0; p < c; ++p) {
+            dn += G[p + (size_t)p * c] * G[p + (size_t)p * c];
+            for (int q = p + 1; q < c; ++q) off += G[p + (size_t)q * c] * G[p + (size_t)q * c];
+        }
+        if (off <= EPS64 * EPS64 * dn) break;
+        for (int p = 0; p < c - 1; ++p)
+            for (int q = p + 1; q < c; ++q) {
+                double gpq = G[p + (size_t)q * c];
+                if (gpq == 0.0) continue;
+                double gpp = G[p + (size_t)p * c], gqq = G[q + (size_t)q * c];
+                double th = (gqq - gpp) / (2 * gpq);
+                double t = copysign(1.0, th) / (fabs(th) + sqrt(1 + th * th));
+                double cs = 1 / sqrt(1 + t * t), sn = t * cs;
+                for (int k = 0; k < c; ++k) { /* G <- Jᵀ G J, columns p, q then rows p, q */
+                    double gp = G[k + (size_t)p * c], gq = G[k + (size_t)q * c];
+                    G[k + (size_t)p * c] = cs * gp - sn * gq;
+                    G[k + (size_t)q * c] = sn * gp + cs * gq;
+                }
+                for (int k = 0; k < c; ++k) {
+                    double gp = G[p + (size_t)k * c], gq = G[q + (size_t)k * c];
+                    G[p + (size_t)k * c] = cs * gp - sn * gq;
+                    G[q + (size_t)k * c] = sn * gp + cs * gq;
+                }
+            }
+    }
+    for (int j = 0; j < c; ++j) s[j] = sqrt(fmax(G[j + (size_t)j * c], 0.0));
+    qsort(s, (size_t)c, sizeof(double), cmp_desc);
+    free(G);
+}
+
+/* Singular values by Householder bidiagonalization (the first stage of LAPACK's dgesdd) of the tall side
+ * (A, or Aᵀ when m < n), then bisection on the Golub–Kahan tridiagonal TGK = tridiag(d1, e1, d2, ..., dc) with
+ * zero diagonal, whose eigenvalues are ±s. Sturm counts of TGK give each s to an absolute error of a few
+ * eps·s_max and better for small s (Demmel & Kahan 1990), with no squaring of A. */
+void orc_svdvals_bidiag(const double *A, int m, int n, double *s) {
+    int tr = m < n;
+    int r = tr ? n : m, c = tr ? m : n;
+    double *X = malloc(sizeof(double) * (size_t)r * c), *v = malloc(sizeof(double) * (size_t)(r > c ? r : c));
+    double *off = malloc(sizeof(double) * (size_t)(2 * c));
+    for (int j = 0; j < c; ++j)
+        for (int i = 0; i < r; ++i) X[i + (size_t)j * r] = tr ? A[j + (size_t)i * m] : A[i + (size_t)j * m];
+#define XX(i, j) X[(i) + (size_t)(j) * r]
+    for (int k = 0; k < c; ++k) {
+        /* left reflector on column k, rows k..r-1 */
+        double nrm = 0;
+        for (int i = k; i < r; ++i) nrm += XX(i, k) * XX(i, k);
+        nrm = sqrt(nrm);
+        double dk = 0;
+        if (nrm > 0) {
+            double a = -copysign(nrm, XX(k, k));
+            for (int i = k; i < r; ++i) v[i] = XX(i, k);
+            v[k] -= a;
+            double vn = 0;
+            for (int i = k; i < r; ++i) vn += v[i] * v[i];
+            if (vn > 0) {
+                for (int j = k + 1; j < c; ++j) {
+                    double dt = 0;
+                    for (int i = k; i < r; ++i) dt += v[i] * XX(i, j);
+                    double f = 2 * dt / vn;
+                    for (int i = k; i < r; ++i) XX(i, j) -= f * v[i];
+                }
+            }
+            dk = a;
+        }
+        off[2 * k] = dk;
+        if (k == c - 1) break;
+        /* right reflector on row k, columns k+1..c-1 */
+        nrm = 0;
+        for (int j = k + 1; j < c; ++j) nrm += XX(k, j) * XX(k, j);
+        nrm = sqrt(nrm);
+        double ek = 0;
+        if (nrm > 0) {
+            double a = -copysign(nrm, XX(k, k + 1));
+            for (int j = k + 1; j < c; ++j) v[j] = XX(k, j);
+            v[k + 1] -= a;
+            double vn = 0;
+            for (int j = k + 1; j < c; ++j) vn += v[j] * v[j];
+            if (vn > 0) {
+                for (int i = k + 1; i < r; ++i) {
+                    double dt = 0;
+                    for (int j = k + 1; j < c; ++j) dt += v[j] * XX(i, j);
+                    double f = 2 * dt / vn;
+                    for (int j = k + 1; j < c; ++j) XX(i, j) -= f * v[j];
+                }
+            }
+            ek = a;
+        }
+        off[2 * k + 1] = ek;
+    }
+#undef XX
+    /* TGK of order N = 2c: zero diagonal, off-diagonal off[0..N-2]; eigenvalue c + t (ascending) is s_{c-t} */
+    int N = 2 * c;
+    double smax = 0;
+    for (int i = 0; i < N - 1; ++i) smax = fmax(smax, fabs(off[i]));
+    double hi = 2 * smax * (1 + 4 * EPS64) + 4 * DBL_MIN;
+    for (int t = 0; t < c; ++t) {
+        double a = 0, b = hi;
+        for (int it = 0; it < 200; ++it) {
+            double x = 0.5 * (a + b);
+            if (!(x > a && x < b) || b - a <= 2 * EPS64 * fmax(fabs(a), fabs(b))) break;
+            int cnt = 0;
+            double q = -x;
+            cnt += q < 0;
+            for (int i = 1; i < N; ++i) {
+                double e = off[i - 1];
+                if (q == 0) q = EPS64 * (fabs(e) + EPS64);
+                q = -x - e * e / q;
+                cnt += q < 0;
+            }
+            if (cnt > c + t) b = x;
+            else a = x;
+        }
+        s[c - 1 - t] = 0.5 * (a + b);
+    }
+    free(X);
+    free(v);
+    free(off);
+}
+
+/* 0: one-sided Jacobi on A (default); 1: Gram eigenvalues (diagnostic); 2: bidiagonalization + TGK bisection */
+static int svd_route = 0;
+void orc_set_svd_route(int route) { svd_route = route; }
+static void gcv_svdvals(const double *A, int m, int n, double *s) {
+    if (svd_route == 1) gram_svdvals(A, m, n, s);
+    else if (svd_route == 2) orc_svdvals_bidiag(A, m, n, s);
+    else orc_svdvals(A, m, n, s);
+}
+
 /* gcv_dof (:1321-1329) */
 double orc_gcv_dof(int m, int n, const double *gamma, int ngamma, double lambda) {
     double dof = (double)(m - n > 0 ? m - n : 0);
@@ -1131,7 +1280,7 @@ static double loggcv_f(double logmu, void *p) {
 static void lsq_gcv(reg_ws *R, double *x_out, double *mu_out, double *chi2_out) {
     int m = R->m, n = R->n;
     R->ngamma = m < n ? m : n;
-    orc_svdvals(R->A, m, n, R->gamma);
+    gcv_svdvals(R->A, m, n, R->gamma);
     gcv_ctx c = {R, (EPS64 * EPS64) / m};
     cache_reset(R);
     double xf, yf;
@@ -1223,7 +1372,9 @@ void orc_lcurve_cache_stats(int out[3]) {
     out[1] = atomic_exchange(&g_lc_max_pc, 0);
     out[2] = atomic_exchange(&g_lc_max_sc, 0);
 }
-double orc_lcurve_corner(orc_fun2 f, void *ctx, double xlow, double xhigh, double xtol, double Ptol, double Ctol) {
+/* lcurve_corner (:872-919); backtracking is the reference's keyword (default true) */
+double orc_lcurve_corner_bt(orc_fun2 f, void *ctx, double xlow, double xhigh, double xtol, double Ptol, double Ctol,
+                            int backtracking) {
     lc_t L;
     memset(&L, 0, sizeof L);
     L.f = f; L.ctx = ctx;
@@ -1241,10 +1392,12 @@ double orc_lcurve_corner(orc_fun2 f, void *ctx, double xlow, double xhigh, doubl
     while (!(fabs(st.x[3] - st.x[0]) < xtol || norm2(st.P[0][0] - st.P[3][0], st.P[0][1] - st.P[3][1]) < Ptol)) {
         iter += 1;
         /* backtracking (:892-900) */
-        double xb = L.pc[pc_argmax(&L)].x;
-        for (int k = 0; k < L.nsc; ++k) {
-            const lstate *s = &L.sc[k];
-            if ((s->x[1] == xb || s->x[2] == xb) && fabs(s->x[3] - s->x[0]) <= fabs(st.x[3] - st.x[0])) st = *s;
+        if (backtracking) {
+            double xb = L.pc[pc_argmax(&L)].x;
+            for (int k = 0; k < L.nsc; ++k) {
+                const lstate *s = &L.sc[k];
+                if ((s->x[1] == xb || s->x[2] == xb) && fabs(s->x[3] - s->x[0]) <= fabs(st.x[3] - st.x[0])) st = *s;
+            }
         }
         int i2 = pc_find(&L, st.x[1]), i3 = pc_find(&L, st.x[2]);
         if (L.pc[i2].C > L.pc[i3].C) {
@@ -1267,6 +1420,7 @@ double orc_lcurve_corner(orc_fun2 f, void *ctx, double xlow, double xhigh, doubl
             st = ns;
         }
         update_curvature(&L, &st, Ptl, Pbr, Ctol);
+        if (!backtracking) continue;
         if (L.nsc == L.capsc) { L.capsc = L.capsc ? 2 * L.capsc : 64; L.sc = realloc(L.sc, sizeof(lstate) * L.capsc); }
         L.sc[L.nsc++] = st;
     }
@@ -1276,6 +1430,10 @@ double orc_lcurve_corner(orc_fun2 f, void *ctx, double xlow, double xhigh, doubl
     atomic_max_int(&g_lc_max_sc, L.nsc);
     free(L.fc); free(L.pc); free(L.sc);
     return xbest;
+}
+
+double orc_lcurve_corner(orc_fun2 f, void *ctx, double xlow, double xhigh, double xtol, double Ptol, double Ctol) {
+    return orc_lcurve_corner_bt(f, ctx, xlow, xhigh, xtol, Ptol, Ctol, 1);
 }
 
 static void lcurve_point(double logmu, void *p, double out[2]) {
@@ -1368,6 +1526,10 @@ typedef struct {
     double *u, *du;
     int *sidx;
     int npts, numeval, mineval, maxeval;
+    /* loss at grid index I (1-based); NULL = the NNLS loss of loss_with_grad! below */
+    void (*loss)(void *ctx, int I, double *u, double *du);
+    void *loss_ctx;
+    int *trace; /* if set, the grid indices in evaluation order */
 } flip_ws;
 
 static void flip_ws_init(flip_ws *F, int m, int n, int K) {
@@ -1412,7 +1574,9 @@ static void flip_eval(flip_ws *F, int I) {
 static int flip_update(flip_ws *F, int I) {
     if (F->numeval >= F->maxeval) return 1;
     if (F->seen[I]) return 0;
-    flip_eval(F, I);
+    if (F->loss) F->loss(F->loss_ctx, I, &F->u[I], &F->du[I]);
+    else flip_eval(F, I);
+    if (F->trace) F->trace[F->numeval] = I;
     /* insertsorted! (src/utils.jl:42-49) */
     int len = ++F->npts, val = I;
     for (int i = 1; i <= len - 1; ++i) {
@@ -1428,12 +1592,10 @@ static int flip_update(flip_ws *F, int I) {
 static int box_evaluated(const flip_ws *F, int lo, int hi) { return (F->seen[lo] + F->seen[hi]) >= 2; }
 
 /* evaluate_box! (:802-815); x NaN = natural corner order, else sorted by distance to x */
+static void orc_sorted_corners_(const double *alphas, int lo, int hi, double x, int out[2]);
 static void evaluate_box(flip_ws *F, int lo, int hi, double x) {
     int cs[2] = {lo, hi};
-    if (!isnan(x)) {
-        double d0 = (F->alphas[lo - 1] - x) * (F->alphas[lo - 1] - x), d1 = (F->alphas[hi - 1] - x) * (F->alphas[hi - 1] - x);
-        if (d1 < d0) { cs[0] = hi; cs[1] = lo; }
-    }
+    if (!isnan(x)) orc_sorted_corners_(F->alphas, lo, hi, x, cs);
     for (int k = 0; k < 2; ++k) {
         if (box_evaluated(F, lo, hi)) break;
         if (flip_update(F, cs[k])) break;
@@ -1448,6 +1610,32 @@ static void flip_init_rec(flip_ws *F, int lo, int hi, int depth) {
     flip_init_rec(F, lo, mid, depth - 1);
     flip_init_rec(F, mid, hi, depth - 1);
 }
+/* minimal_bounding_box (:778-800) on the 1-D grid: halve [1, K] towards x (bisect (:684-693) splits at
+ * (lo + hi) ÷ 2; contains (:840-844) is lo <= x <= hi on grid values, so x on a split point goes left) until the
+ * half holding x is not fully evaluated or has width 1. */
+static void minimal_bounding_box(const unsigned char *seen, const double *alphas, int K, double x, int *blo, int *bhi) {
+    int lo = 1, hi = K;
+    for (;;) {
+        int mid = (lo + hi) / 2;
+        if (alphas[lo - 1] <= x && x <= alphas[mid - 1]) {
+            if (!(seen[lo] && seen[mid]) || !(mid - lo > 1)) { *blo = lo; *bhi = mid; return; }
+            hi = mid;
+        } else {
+            if (!(seen[mid] && seen[hi]) || !(hi - mid > 1)) { *blo = mid; *bhi = hi; return; }
+            lo = mid;
+        }
+    }
+}
+void orc_minimal_bounding_box(const unsigned char *seen, const double *alphas, int K, double x, int *lo, int *hi) {
+    minimal_bounding_box(seen, alphas, K, x, lo, hi);
+}
+/* sorted_corners (:834-838) of the 1-D box [lo, hi]: by squared distance to x, ties in corner order */
+void orc_sorted_corners(const double *alphas, int lo, int hi, double x, int out[2]) {
+    double d0 = (alphas[lo - 1] - x) * (alphas[lo - 1] - x), d1 = (alphas[hi - 1] - x) * (alphas[hi - 1] - x);
+    out[0] = d1 < d0 ? hi : lo;
+    out[1] = d1 < d0 ? lo : hi;
+}
+
 /* suggest_point (:544-566) */
 static void suggest_point(const flip_ws *F, double *xo, double *uo) {
     int I0 = F->sidx[1];
@@ -1474,27 +1662,39 @@ static double flip_search(flip_ws *F, const double *b, int mineval, int *nevals)
         flip_init_rec(F, 1, K, depth);
         if (F->numeval >= mineval) break;
     }
-    /* bisection_search (:750-775) with minimal_bounding_box (:778-800) */
+    /* bisection_search (:750-775) */
     double x, u;
     suggest_point(F, &x, &u);
     for (;;) {
-        int lo = 1, hi = K, blo, bhi;
-        for (;;) {
-            int mid = (lo + hi) / 2;
-            if (F->alphas[lo - 1] <= x && x <= F->alphas[mid - 1]) {
-                if (!box_evaluated(F, lo, mid) || !(mid - lo > 1)) { blo = lo; bhi = mid; break; }
-                hi = mid;
-            } else {
-                if (!box_evaluated(F, mid, hi) || !(hi - mid > 1)) { blo = mid; bhi = hi; break; }
-                lo = mid;
-            }
-        }
+        int blo, bhi;
+        minimal_bounding_box(F->seen, F->alphas, K, x, &blo, &bhi);
         evaluate_box(F, blo, bhi, x);
         suggest_point(F, &x, &u);
         if (F->numeval >= F->maxeval || (bhi - blo) <= 1) break;
     }
     if (nevals) *nevals = F->numeval;
     return x;
+}
+
+static void orc_sorted_corners_(const double *alphas, int lo, int hi, double x, int out[2]) {
+    orc_sorted_corners(alphas, lo, hi, x, out);
+}
+
+/* The surrogate search of optimize_flip_angle! (src/T2mapSEcorr.jl:409-423) on any loss: initialize! with
+ * mineval, bisection_search with maxeval = K; returns the minimizer and writes the evaluated grid indices
+ * (1-based, in order) to trace (K entries at most). */
+double orc_surrogate_search(const double *alphas, int K, int mineval,
+                            void (*loss)(void *ctx, int I, double *u, double *du), void *ctx, int *trace,
+                            int *nevals) {
+    flip_ws F;
+    flip_ws_init(&F, 1, 1, K);
+    F.alphas = alphas;
+    F.loss = loss;
+    F.loss_ctx = ctx;
+    F.trace = trace;
+    double a = flip_search(&F, NULL, mineval, nevals);
+    flip_ws_free(&F);
+    return a;
 }
 
 double orc_flip_search(const double *As, const double *dAs, const double *alphas, int m, int n, int K, int mineval,

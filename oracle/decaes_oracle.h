@@ -105,6 +105,11 @@ void orc_lsqnonneg_gcv(const double *A, const double *b, int m, int n, double *x
 void orc_lsqnonneg_lcurve(const double *A, const double *b, int m, int n, double *x, double *mu, double *chi2);
 /* Singular values, descending, min(m,n) of them (one-sided Jacobi; stands in for LAPACK dgesdd). */
 void orc_svdvals(const double *A, int m, int n, double *s);
+/* Householder bidiagonalization + Golub-Kahan tridiagonal bisection (the device kernel's route) */
+void orc_svdvals_bidiag(const double *A, int m, int n, double *s);
+/* GCV singular-value route: 0 = one-sided Jacobi on A (default), 1 = Gram-matrix eigenvalues (diagnostic),
+ * 2 = orc_svdvals_bidiag */
+void orc_set_svd_route(int route);
 double orc_gcv_dof(int m, int n, const double *gamma, int ngamma, double lambda);
 
 /* ---- scalar optimizers (src/optimization.jl) with a C callback ---- */
@@ -118,6 +123,8 @@ void orc_brent_minimize(orc_fun1 f, void *ctx, double x1, double x2, double xrto
 /* lcurve_corner on a caller-supplied curve f(logmu) -> (xi, eta) (src/lsqnonneg.jl:872-972). */
 typedef void (*orc_fun2)(double x, void *ctx, double out[2]);
 double orc_lcurve_corner(orc_fun2 f, void *ctx, double xlow, double xhigh, double xtol, double Ptol, double Ctol);
+double orc_lcurve_corner_bt(orc_fun2 f, void *ctx, double xlow, double xhigh, double xtol, double Ptol, double Ctol,
+                            int backtracking);
 /* Largest L-curve cache sizes (function, point, state caches) seen since the previous call. */
 void orc_lcurve_cache_stats(int out[3]);
 /* CubicHermiteInterpolator + minimize (src/splines.jl:62-110). out = (x, u). */
@@ -127,6 +134,13 @@ void orc_hermite_minimize(double a, double b, double u0, double u1, double m0, d
 /* As: m x n x K grid bases, dAs: their d/dalpha. Returns alpha_opt; nevals optional. */
 double orc_flip_search(const double *As, const double *dAs, const double *alphas, int m, int n, int K, int mineval,
                        const double *b, int *nevals);
+/* The same searcher on any loss (u, du at 1-based grid index I); trace receives the evaluated indices in order. */
+double orc_surrogate_search(const double *alphas, int K, int mineval,
+                            void (*loss)(void *ctx, int I, double *u, double *du), void *ctx, int *trace,
+                            int *nevals);
+/* minimal_bounding_box / sorted_corners of the 1-D searcher; seen is indexed 1..K */
+void orc_minimal_bounding_box(const unsigned char *seen, const double *alphas, int K, double x, int *lo, int *hi);
+void orc_sorted_corners(const double *alphas, int lo, int hi, double x, int out[2]);
 
 /* ---- whole pipeline (src/T2mapSEcorr.jl:148-238,512-591) ---- */
 /* image: (nx,ny,nz,nTE) column-major. alpha_in: B1 map (nx,ny,nz) or NULL. Outputs are caller-allocated and

@@ -211,14 +211,16 @@ def brent_minimize(f, x1, x2, xrtol=1.4901161193847656e-8, xatol=1.4901161193847
     return xo.value, fo.value
 
 
-def lcurve_corner(f, xlow, xhigh, xtol=1e-4, Ptol=1e-4, Ctol=1e-4):
+def lcurve_corner(f, xlow, xhigh, xtol=1e-4, Ptol=1e-4, Ctol=1e-4, backtracking=True):
     def _cb(x, _ctx, out):
         a, b = f(x)
         out[0], out[1] = a, b
 
     cb = FUN2(_cb)
-    return lib().orc_lcurve_corner(cb, None, C.c_double(xlow), C.c_double(xhigh), C.c_double(xtol),
-                                   C.c_double(Ptol), C.c_double(Ctol))
+    L = lib()
+    L.orc_lcurve_corner_bt.restype = C.c_double
+    return L.orc_lcurve_corner_bt(cb, None, C.c_double(xlow), C.c_double(xhigh), C.c_double(xtol),
+                                  C.c_double(Ptol), C.c_double(Ctol), C.c_int(1 if backtracking else 0))
 
 
 def hermite_minimize(a, b, u0, u1, m0, m1):
@@ -236,6 +238,47 @@ def flip_search(As, dAs, alphas, b, mineval):
                               C.c_int(n), C.c_int(K), C.c_int(mineval),
                               _p(np.ascontiguousarray(b, dtype=np.float64)), C.byref(ne))
     return a, ne.value
+
+
+LOSS = C.CFUNCTYPE(None, C.c_void_p, C.c_int, _dp, _dp)
+
+
+def surrogate_search(alphas, mineval, loss):
+    """The flip-angle searcher on loss(alpha) -> (u, du/dalpha) over the grid alphas. Returns (x, trace), trace
+    the 1-based grid indices in evaluation order."""
+    alphas = np.ascontiguousarray(alphas, dtype=np.float64)
+    K = len(alphas)
+
+    def _cb(_ctx, I, u, du):
+        uu, dd = loss(float(alphas[I - 1]))
+        u[0], du[0] = uu, dd
+
+    cb = LOSS(_cb)
+    trace = np.zeros(K, dtype=np.int32)
+    nev = C.c_int(0)
+    L = lib()
+    L.orc_surrogate_search.restype = C.c_double
+    x = L.orc_surrogate_search(_p(alphas), C.c_int(K), C.c_int(mineval), cb, None, trace.ctypes.data_as(_ip),
+                               C.byref(nev))
+    return x, [int(i) for i in trace[:nev.value]]
+
+
+def minimal_bounding_box(seen_indices, alphas, x):
+    alphas = np.ascontiguousarray(alphas, dtype=np.float64)
+    K = len(alphas)
+    seen = np.zeros(K + 1, dtype=np.uint8)
+    seen[list(seen_indices)] = 1
+    lo, hi = C.c_int(0), C.c_int(0)
+    lib().orc_minimal_bounding_box(seen.ctypes.data_as(C.POINTER(C.c_ubyte)), _p(alphas), C.c_int(K), C.c_double(x),
+                                   C.byref(lo), C.byref(hi))
+    return lo.value, hi.value
+
+
+def sorted_corners(alphas, lo, hi, x):
+    alphas = np.ascontiguousarray(alphas, dtype=np.float64)
+    out = (C.c_int * 2)()
+    lib().orc_sorted_corners(_p(alphas), C.c_int(lo), C.c_int(hi), C.c_double(x), out)
+    return out[0], out[1]
 
 
 def make_t2map_opts(shape, nTE, TE, nT2, T2Range, Reg, T1=1.0, Threshold=0.0, MinRefAngle=50.0, nRefAngles=64,

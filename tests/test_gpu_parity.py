@@ -1,21 +1,21 @@
 """GPU parity: the gfx950 path through the C ABI against the FP64 oracle on identical seeded inputs.
 
-Tolerances are the plain FP64 path's: alpha <= 0.01 deg, distribution relative L2 <= 1e-6, MWF (sfr) absolute
-<= 1e-6 per voxel. Both sides reduce in different orders (the reference's @simd reductions reassociate too), so
-a voxel whose flip-angle bisection or regularization search sits on a branch tie may take the other branch; the
-tests bound the fraction of such voxels and print the full error distribution.
+Every voxel is checked. The bar of the plain FP64 path (alpha <= 0.01 deg, distribution relative L2 <= 1e-6, MWF
+(sfr) <= 1e-6, derived maps <= 1e-6 relative) holds on every voxel that the oracle itself computes stably at the
+rounding level; voxels whose regularization search ends on a rounding-level tie are listed by id and bounded by the
+oracle's own spread under ulp-scaled inputs (parity_util).
 """
 import json
-import math
 import os
-from pathlib import Path
 
 import numpy as np
 import pytest
 
+import parity_util as PU
+
 pytestmark = pytest.mark.gpu
 
-OUT = Path(os.environ.get("GRAFT_REPO_ROOT", Path(__file__).resolve().parents[1])) / "gpurun_out"
+OUT = PU.OUT
 
 CASES = {
     # name: (shape, nTE, TE, nT2, T2Range, Reg, nRefAngles, extra options, ncomp, mask fraction)
@@ -28,6 +28,12 @@ CASES = {
 }
 
 
+def _oracle_opts(oracle, shape, m, TE, n, T2R, reg, K, Kmin, extra, over):
+    return oracle.make_t2map_opts(shape, m, TE, n, T2R, reg, nRefAngles=K, nRefAnglesMin=Kmin,
+                                  Chi2Factor=extra.get("Chi2Factor"), NoiseLevel=extra.get("NoiseLevel"),
+                                  RefConAngle=over.get("RefConAngle", 180.0), SetFlipAngle=over.get("SetFlipAngle"))
+
+
 def _run(dmi, oracle, name, seed=11, **over):
     shape, m, TE, n, T2R, reg, K, extra, ncomp, frac = CASES[name]
     img, nin = dmi.synth_mse(shape, m, TE, ncomp=ncomp, mask_fraction=frac, seed=seed)
@@ -36,76 +42,37 @@ def _run(dmi, oracle, name, seed=11, **over):
     kw.update(over)
     opts = dmi.T2mapOptions.from_image(img, **kw)
     maps, dist = dmi.T2mapSEcorr(img, opts)
-    oo = oracle.make_t2map_opts(shape, m, TE, n, T2R, reg, nRefAngles=K, nRefAnglesMin=opts.nRefAnglesMin,
-                                Chi2Factor=extra.get("Chi2Factor"), NoiseLevel=extra.get("NoiseLevel"),
-                                RefConAngle=over.get("RefConAngle", 180.0), SetFlipAngle=over.get("SetFlipAngle"))
+    oo = _oracle_opts(oracle, shape, m, TE, n, T2R, reg, K, opts.nRefAnglesMin, extra, over)
     omaps, odist, nv = oracle.t2map(img, oo, nthreads=os.cpu_count())
-    return img, opts, maps, dist, omaps, odist, nv
+    return img, opts, oo, maps, dist, omaps, odist, nv
 
 
-def _errors(maps, dist, omaps, odist, T2R):
-    mask = ~np.isnan(omaps["gdn"])
-    d_g = dist[mask]
-    d_o = odist[mask]
-    rel = np.linalg.norm(d_g - d_o, axis=1) / np.maximum(np.linalg.norm(d_o, axis=1), 1e-300)
-    da = np.abs(maps["alpha"][mask] - omaps["alpha"][mask])
-    return mask, rel, da
-
-
-def _summary(name, mask, rel, da, mwf_err, maps, omaps):
-    def q(x):
-        x = np.asarray(x)
-        return dict(max=float(np.max(x)), p99=float(np.quantile(x, 0.99)), median=float(np.median(x)))
-
-    s = dict(case=name, voxels=int(mask.sum()), dist_rel_l2=q(rel), alpha_abs=q(da), mwf_abs=q(mwf_err),
-             mu_rel=q(np.abs(maps["mu"][mask] - omaps["mu"][mask]) / np.maximum(np.abs(omaps["mu"][mask]), 1e-300)),
-             frac_ok=float(np.mean((rel <= 1e-6) & (da <= 1e-2) & (mwf_err <= 1e-6))))
-    OUT.mkdir(exist_ok=True)
-    with open(OUT / f"parity_{name}.json", "w") as f:
-        json.dump(s, f, indent=1)
-    print(json.dumps(s), flush=True)
-    return s
+def _curves_check(maps, omaps, sel):
+    def f(stable):
+        for k in ("resnorm", "chi2factor"):
+            a, b = maps[k][sel][stable], omaps[k][sel][stable]
+            err = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+            assert np.all(err <= PU.MAPS_REL), (k, float(err.max(initial=0.0)))
+        a, b = maps["decaycurve"][sel][stable], omaps["decaycurve"][sel][stable]
+        err = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+        assert np.all(err <= PU.MAPS_REL), ("decaycurve", float(err.max(initial=0.0)))
+    return f
 
 
 @pytest.mark.parametrize("name", list(CASES))
 def test_t2map_matches_oracle(dmi, oracle, name):
     print(f"[parity] case={name}", flush=True)
-    img, opts, maps, dist, omaps, odist, nv = _run(dmi, oracle, name)
-    shape, m, TE, n, T2R = CASES[name][:5]
+    img, opts, oo, maps, dist, omaps, odist, nv = _run(dmi, oracle, name)
+    T2R = CASES[name][4]
     # identical voxel selection and NaN placement
     assert np.array_equal(np.isnan(maps["gdn"]), np.isnan(omaps["gdn"]))
     assert np.array_equal(np.isnan(dist), np.isnan(odist))
-    mask, rel, da = _errors(maps, dist, omaps, odist, T2R)
-    pg = oracle.t2part(dist, T2R, (T2R[0], 40e-3), (40e-3, T2R[1]))
-    po = oracle.t2part(odist, T2R, (T2R[0], 40e-3), (40e-3, T2R[1]))
-    mwf_err = np.abs(pg["sfr"][mask] - po["sfr"][mask])
-    s = _summary(name, mask, rel, da, mwf_err, maps, omaps)
-    # the flip-angle search runs before regularization: alpha always agrees to the FP64 bar
-    assert np.max(da) <= 1e-2, s
-    if opts.Reg == "lcurve":
-        # The L-curve corner search is discontinuous at the rounding level: its isapprox-keyed caches, backtracking
-        # and curvature comparisons pick a corner up to ~4 xtol away after a 1-ulp change of the input. The bar is
-        # therefore the oracle's own sensitivity: perturb the input by 1 ulp, rerun the oracle, and require the GPU
-        # to differ from the oracle no more often and no further than the oracle differs from itself.
-        img2 = img * (1 + 2.0 ** -52)
-        oo = oracle.make_t2map_opts(shape, m, TE, n, T2R, "lcurve", nRefAngles=opts.nRefAngles)
-        pmaps, pdist, _ = oracle.t2map(img2, oo, nthreads=os.cpu_count())
-        self_rel = np.linalg.norm(pdist[mask] - odist[mask], axis=1) / np.linalg.norm(odist[mask], axis=1)
-        self_dlog = np.abs(np.log(pmaps["mu"][mask]) - np.log(omaps["mu"][mask]))
-        dlog = np.abs(np.log(maps["mu"][mask]) - np.log(omaps["mu"][mask]))
-        print(f"[parity] lcurve self-sensitivity: frac dist>1e-6 {np.mean(self_rel > 1e-6):.4f}, "
-              f"max dlogmu {self_dlog.max():.2e}; gpu: frac {np.mean(rel > 1e-6):.4f}, max dlogmu {dlog.max():.2e}")
-        assert np.mean(rel > 1e-6) <= np.mean(self_rel > 1e-6) + 0.05, s
-        assert dlog.max() <= max(2 * self_dlog.max(), 1e-3), s
-        assert np.max(mwf_err) <= 1e-3, s
-    else:
-        assert s["frac_ok"] >= 0.97, s
-    # per-voxel maps that depend only on the solution agree to the same tolerance (L-curve: to its own
-    # rounding-level sensitivity, bounded above by the mixed-precision bar of 1e-3)
-    tol = 1e-3 if opts.Reg == "lcurve" else 1e-6
-    for k in ("gdn", "fnr", "snr", "resnorm", "chi2factor"):
-        a, b = maps[k][mask], omaps[k][mask]
-        assert np.mean(np.abs(a - b) <= tol * np.maximum(np.abs(b), 1.0)) >= 0.97, k
+    _, _, sel, spread = PU.oracle_ensemble(oracle, img, oo, T2R)
+    s, stable = PU.check(name, oracle, T2R, sel, spread, maps, dist, omaps, odist,
+                         extra_check=_curves_check(maps, omaps, sel))
+    # the flip-angle search and the unregularized/chi2/mdp solves are rounding-stable on every voxel here
+    if opts.Reg in ("none", "chi2", "mdp"):
+        assert stable.all(), s
 
 
 def test_decaybasisset_matches_oracle(dmi, oracle):
@@ -121,17 +88,21 @@ def test_decaybasisset_matches_oracle(dmi, oracle):
 
 def test_general_refcon_angle(dmi, oracle):
     print("[parity] case=refcon172", flush=True)
-    img, opts, maps, dist, omaps, odist, nv = _run(dmi, oracle, "none", RefConAngle=172.0)
-    mask, rel, da = _errors(maps, dist, omaps, odist, (10e-3, 2.0))
-    assert np.mean((rel <= 1e-6) & (da <= 1e-2)) >= 0.97
+    img, opts, oo, maps, dist, omaps, odist, nv = _run(dmi, oracle, "none", RefConAngle=172.0)
+    _, _, sel, spread = PU.oracle_ensemble(oracle, img, oo, (10e-3, 2.0))
+    s, stable = PU.check("refcon172", oracle, (10e-3, 2.0), sel, spread, maps, dist, omaps, odist,
+                         extra_check=_curves_check(maps, omaps, sel))
+    assert stable.all(), s
 
 
 def test_set_flip_angle(dmi, oracle):
     print("[parity] case=setflip", flush=True)
-    img, opts, maps, dist, omaps, odist, nv = _run(dmi, oracle, "c1_chi2", SetFlipAngle=165.0)
-    mask, rel, da = _errors(maps, dist, omaps, odist, (10e-3, 2.0))
-    assert np.all(maps["alpha"][mask] == 165.0)
-    assert np.mean(rel <= 1e-6) >= 0.97
+    img, opts, oo, maps, dist, omaps, odist, nv = _run(dmi, oracle, "c1_chi2", SetFlipAngle=165.0)
+    _, _, sel, spread = PU.oracle_ensemble(oracle, img, oo, (10e-3, 2.0))
+    assert np.all(maps["alpha"][sel] == 165.0)
+    s, stable = PU.check("setflip165", oracle, (10e-3, 2.0), sel, spread, maps, dist, omaps, odist,
+                         extra_check=_curves_check(maps, omaps, sel))
+    assert stable.all(), s
 
 
 def test_b1_map(dmi, oracle):
@@ -139,14 +110,15 @@ def test_b1_map(dmi, oracle):
     shape, m, TE, n, T2R = (8, 8, 2), 32, 10e-3, 40, (10e-3, 2.0)
     img, _ = dmi.synth_mse(shape, m, TE, mask_fraction=0.7, seed=5)
     b1 = np.asfortranarray(np.random.default_rng(0).uniform(120, 180, shape))
-    opts = dmi.T2mapOptions.from_image(img, TE=TE, nT2=n, T2Range=T2R, Reg="none", nRefAngles=32, Silent=True)
+    opts = dmi.T2mapOptions.from_image(img, TE=TE, nT2=n, T2Range=T2R, Reg="none", nRefAngles=32, Silent=True,
+                                       SaveRegParam=True)
     maps, dist = dmi.T2mapSEcorr(img, opts, alpha=b1)
     oo = oracle.make_t2map_opts(shape, m, TE, n, T2R, "none", nRefAngles=32)
-    omaps, odist, _ = oracle.t2map(img, oo, alpha_in=b1)
+    omaps, odist, sel, spread = PU.oracle_ensemble(oracle, img, oo, T2R, alpha_in=b1)
     # alpha echoes the B1 map everywhere (also below threshold, as load_B1map! wrote it)
     assert np.array_equal(maps["alpha"], b1)
-    mask, rel, da = _errors(maps, dist, omaps, odist, T2R)
-    assert np.mean(rel <= 1e-6) >= 0.97
+    s, stable = PU.check("b1map", oracle, T2R, sel, spread, maps, dist, omaps, odist)
+    assert stable.all(), s
 
 
 def test_t2part_matches_oracle(dmi, oracle):
@@ -182,8 +154,10 @@ def test_mixed_precision_error(dmi, oracle, name):
     alpha <= 0.1 deg and MWF <= 1e-3 absolute per voxel; every output's error distribution is written out."""
     print(f"[parity] case=mixed_{name}", flush=True)
     shape, m, TE, n, T2R, reg, K, extra, ncomp, frac = CASES[name]
-    img, opts, maps, dist, omaps, odist, nv = _run(dmi, oracle, name, precision="mixed")
-    mask, rel, da = _errors(maps, dist, omaps, odist, T2R)
+    img, opts, oo, maps, dist, omaps, odist, nv = _run(dmi, oracle, name, precision="mixed")
+    mask = ~np.isnan(omaps["gdn"])
+    rel = np.linalg.norm(dist[mask] - odist[mask], axis=1) / np.linalg.norm(odist[mask], axis=1)
+    da = np.abs(maps["alpha"][mask] - omaps["alpha"][mask])
     pg = oracle.t2part(dist, T2R, (T2R[0], 40e-3), (40e-3, T2R[1]))
     po = oracle.t2part(odist, T2R, (T2R[0], 40e-3), (40e-3, T2R[1]))
     mwf = np.abs(pg["sfr"][mask] - po["sfr"][mask])
@@ -220,15 +194,11 @@ def test_size_limits(dmi, oracle, shape, m, n, K, reg):
     maps, dist = dmi.T2mapSEcorr(img, opts)
     oo = oracle.make_t2map_opts(shape, m, TE, n, T2R, reg, nRefAngles=K, nRefAnglesMin=min(5, K),
                                 Chi2Factor=extra.get("Chi2Factor"), NoiseLevel=extra.get("NoiseLevel"))
-    omaps, odist, nv = oracle.t2map(img, oo, nthreads=os.cpu_count())
+    omaps, odist, sel, spread = PU.oracle_ensemble(oracle, img, oo, T2R)
     assert np.array_equal(np.isnan(dist), np.isnan(odist))
-    mask, rel, da = _errors(maps, dist, omaps, odist, T2R)
-    print(f"[parity] limits: {mask.sum()} voxels, max dist rel {rel.max():.2e}, max alpha {da.max():.2e}", flush=True)
-    assert np.max(da) <= 1e-2
-    if reg == "lcurve":
-        assert np.mean(rel <= 1e-6) >= 0.5
-    else:
-        assert np.mean(rel <= 1e-6) >= 0.9
+    s, stable = PU.check(f"limits_m{m}_n{n}_K{K}_{reg}", oracle, T2R, sel, spread, maps, dist, omaps, odist)
+    if reg in ("none", "chi2", "mdp"):
+        assert stable.all(), s
 
 
 @pytest.mark.parametrize("m,reg,setflip", [(33, "gcv", None), (25, "chi2", 165.0)])
@@ -248,13 +218,11 @@ def test_odd_echo_count_basis_layout(dmi, oracle, m, reg, setflip):
     maps, dist = dmi.T2mapSEcorr(img, opts)
     oo = oracle.make_t2map_opts(shape, m, TE, n, T2R, reg, nRefAngles=K, nRefAnglesMin=5,
                                 Chi2Factor=extra.get("Chi2Factor"), SetFlipAngle=setflip)
-    omaps, odist, nv = oracle.t2map(img, oo, nthreads=os.cpu_count())
+    omaps, odist, sel, spread = PU.oracle_ensemble(oracle, img, oo, T2R)
     assert np.array_equal(np.isnan(dist), np.isnan(odist))
-    mask, rel, da = _errors(maps, dist, omaps, odist, T2R)
-    print(f"[parity] odd_m: {mask.sum()} voxels, max dist rel {rel.max():.2e}, max alpha {da.max():.2e}", flush=True)
+    mask = sel
     assert mask.sum() == np.prod(shape)
-    assert np.max(da) <= 1e-2
-    assert np.mean(rel <= 1e-6) >= 0.9
+    PU.check(f"odd_m{m}_{reg}", oracle, T2R, sel, spread, maps, dist, omaps, odist)
     if setflip is not None:
         # one shared basis (m, n), copied out of the padded device layout
         assert maps["decaybasis"].shape == (m, n)
@@ -320,15 +288,7 @@ def test_full_volume_sampled(dmi, oracle):
     lin = np.flatnonzero(inmask.ravel(order="F"))
     vox = np.sort(rng.choice(lin, size=3000, replace=False))
     oo = oracle.make_t2map_opts(shape, m, TE, n, T2R, "gcv", nRefAngles=32, nRefAnglesMin=opts.nRefAnglesMin)
-    omaps, odist = oracle.t2map_subset(img, oo, vox, nthreads=min(os.cpu_count() or 1, 16))
-    sel = np.zeros(inmask.size, bool)
-    sel[vox] = True
-    sel = sel.reshape(inmask.shape, order="F")
-    rel = np.linalg.norm(dist[sel] - odist[sel], axis=1) / np.linalg.norm(odist[sel], axis=1)
-    da = np.abs(maps["alpha"][sel] - omaps["alpha"][sel])
-    po = oracle.t2part(odist, T2R, (T2R[0], 40e-3), (40e-3, T2R[1]))
-    mwf_err = np.abs(parts["sfr"][sel] - po["sfr"][sel])
-    s = _summary("c2_full_volume_sampled", sel, rel, da, mwf_err, maps, omaps)
-    assert np.max(da) <= 1e-2, s
-    assert s["frac_ok"] >= 0.97, s
-    assert np.mean(np.abs(maps["mu"][sel] - omaps["mu"][sel]) <= 1e-6 * omaps["mu"][sel]) >= 0.97, s
+    omaps, odist, sel, spread = PU.oracle_ensemble(oracle, img, oo, T2R, vox=vox)
+    s, stable = PU.check("c2_full_volume_sampled", oracle, T2R, sel, spread, maps, dist, omaps, odist)
+    # rounding-level ties of the GCV search are rare
+    assert (~stable).sum() <= 0.02 * stable.size, s
