@@ -29,6 +29,7 @@ struct dmi_ctx {
     dmi_stats stats{};
     dmi_host::Grids grids;
     dmi_host::LaunchBufs lb;
+    double *noalloc = nullptr;  // this device's address of the static scratch (dmi_t2map_f64_device_noalloc)
     DevBuf vox;
     // host-array staging (FP64 working copies, FP32 transfer buffers)
     DevBuf img, ain, out[12], t2p[5];
@@ -46,55 +47,61 @@ int fail(dmi_ctx *c, int code, const char *msg) {
 
 #define HIPCHECK(c, expr) DMI_HIP((c)->err, expr)
 
-int run_t2map(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, const double *d_alpha_in,
-              const dmi_t2map_out *d) {
+// Device memory one T2mapSEcorr call works in: the grids of its options, a voxel list of vox_cap entries, the
+// launch counters and the per-workgroup slots of the voxel kernel.
+struct RunBufs {
+    dmi_host::GridPtrs g;
+    int64_t *vox;
+    int64_t vox_cap;
+    unsigned *counter;
+    unsigned long long *counters;
+    int *status;
+    double *slots;
+    int64_t slot_doubles;
+};
+
+// Voxel selection and the voxel kernel over [0, V) in chunks of vox_cap voxels (one chunk when the list holds the
+// whole volume); allocates nothing.
+int run_t2map_on(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, const double *d_alpha_in,
+                 const dmi_t2map_out *d, const RunBufs &rb) {
     const int m = o->nTE, n = o->nT2;
     const int64_t V = o->MatrixSize[0] * o->MatrixSize[1] * o->MatrixSize[2];
-    int rc = dmi_host::validate_outputs(c->err, o, d);
-    if (rc) return rc;
     const bool fixed = !std::isnan(o->SetFlipAngle);
-    HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
-    rc = c->grids.ensure(c->err, o, c->stream);
-    if (rc) return rc;
     // the shared basis of SetFlipAngle, nTE x nT2 column-major (the column-major copy without its column padding)
     if (fixed && o->SaveNNLSBasis)
-        HIPCHECK(c, hipMemcpy2DAsync(d->decaybasis, sizeof(double) * m, c->grids.AfixedT.p,
-                                     sizeof(double) * dmi::basis_ld(m), sizeof(double) * m, n, hipMemcpyDeviceToDevice,
-                                     c->stream));
-    HIPCHECK(c, c->vox.ensure(sizeof(int64_t) * std::max<int64_t>(V, 1)));
-    HIPCHECK(c, c->lb.counter.ensure(sizeof(unsigned) * 4));
-    HIPCHECK(c, c->lb.counters.ensure(sizeof(unsigned long long) * 4));
-    HIPCHECK(c, c->lb.status.ensure(sizeof(int) * 4));
-    HIPCHECK(c, hipMemsetAsync(c->lb.counter.p, 0, sizeof(unsigned) * 4, c->stream));
-    HIPCHECK(c, hipMemsetAsync(c->lb.counters.p, 0, sizeof(unsigned long long) * 4, c->stream));
-    HIPCHECK(c, hipMemsetAsync(c->lb.status.p, 0, sizeof(int) * 4, c->stream));
-    HIPCHECK(c, dmi::launch_select(d_image, V, o->Threshold, c->vox.as<int64_t>(), c->lb.counter.as<unsigned>(), c->stream));
-    unsigned nvox = 0;
-    HIPCHECK(c, hipMemcpyAsync(&nvox, c->lb.counter.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
+        HIPCHECK(c, hipMemcpy2DAsync(d->decaybasis, sizeof(double) * m, rb.g.AsT, sizeof(double) * dmi::basis_ld(m),
+                                     sizeof(double) * m, n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(c, dmi::launch_fill(rb.counters, sizeof(unsigned long long) * 4, 0u, c->stream));
+    HIPCHECK(c, dmi::launch_fill(rb.status, sizeof(int) * 4, 0u, c->stream));
     c->stats = dmi_stats{};
-    c->stats.voxels = nvox;
-    if (nvox == 0) {
-        HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
-        HIPCHECK(c, hipEventSynchronize(c->ev[3]));
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[3]);
-        c->stats.total_ms = ms;
-        return DMI_OK;
+    dmi::T2mapParams p = dmi_host::t2map_params(o, rb.g, d_image, d_alpha_in, d);
+    bool launched = false;
+    for (int64_t v0 = 0; v0 < V; v0 += rb.vox_cap) {
+        const int64_t v1 = std::min<int64_t>(V, v0 + rb.vox_cap);
+        HIPCHECK(c, dmi::launch_fill(rb.counter, sizeof(unsigned), 0u, c->stream));
+        HIPCHECK(c, dmi::launch_select(d_image, v0, v1, o->Threshold, rb.vox, rb.counter, c->stream));
+        unsigned nvox = 0;
+        HIPCHECK(c, hipMemcpyAsync(&nvox, rb.counter, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        c->stats.voxels += nvox;
+        if (nvox == 0) continue;
+        if (!launched) HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
+        launched = true;
+        int rc = dmi_host::t2map_launch_raw(c->err, p, rb.vox, nvox, rb.counter, rb.counters, rb.status, rb.slots,
+                                            rb.slot_doubles, c->num_cu, c->stream);
+        if (rc) return rc;
     }
-    dmi::T2mapParams p = dmi_host::t2map_params(o, c->grids, d_image, d_alpha_in, d);
-    HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
-    rc = dmi_host::t2map_launch(c->err, p, c->vox.as<int64_t>(), nvox, c->lb, c->num_cu, c->stream);
-    if (rc) return rc;
-    HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
     unsigned long long cnt[3] = {0, 0, 0};
     int status = 0;
-    HIPCHECK(c, hipMemcpyAsync(cnt, c->lb.counters.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(&status, c->lb.status.p, sizeof status, hipMemcpyDeviceToHost, c->stream));
+    if (launched) {
+        HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
+        HIPCHECK(c, hipMemcpyAsync(cnt, rb.counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipMemcpyAsync(&status, rb.status, sizeof status, hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
     HIPCHECK(c, hipEventSynchronize(c->ev[3]));
     float kms = 0, tms = 0;
-    HIPCHECK(c, hipEventElapsedTime(&kms, c->ev[1], c->ev[2]));
+    if (launched) HIPCHECK(c, hipEventElapsedTime(&kms, c->ev[1], c->ev[2]));
     HIPCHECK(c, hipEventElapsedTime(&tms, c->ev[0], c->ev[3]));
     c->stats.nnls_solves = (int64_t)cnt[1];
     c->stats.fp64_flops = (int64_t)cnt[0];
@@ -104,6 +111,78 @@ int run_t2map(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, const 
     if (status & dmi::kStatusLcurveOverflow)
         return dmi_host::errorf(c->err, DMI_ECAPACITY, "L-curve cache exceeded %d entries for at least one voxel", dmi::kLcurveCap);
     return DMI_OK;
+}
+
+int run_t2map(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, const double *d_alpha_in,
+              const dmi_t2map_out *d) {
+    const int64_t V = o->MatrixSize[0] * o->MatrixSize[1] * o->MatrixSize[2];
+    int rc = dmi_host::validate_outputs(c->err, o, d);
+    if (rc) return rc;
+    HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+    rc = c->grids.ensure(c->err, o, c->stream);
+    if (rc) return rc;
+    HIPCHECK(c, c->vox.ensure(sizeof(int64_t) * std::max<int64_t>(V, 1)));
+    HIPCHECK(c, c->lb.counter.ensure(sizeof(unsigned) * 4));
+    HIPCHECK(c, c->lb.counters.ensure(sizeof(unsigned long long) * 4));
+    HIPCHECK(c, c->lb.status.ensure(sizeof(int) * 4));
+    // slots for the largest persistent grid the voxel kernel can run with these options
+    const int lda = dmi::t2map_lda(o->nTE, o->nT2);
+    const int K = std::isnan(o->SetFlipAngle) ? o->nRefAngles : 1;
+    int per_cu = 0;
+    HIPCHECK(c, dmi::t2map_occupancy(o->Reg, &per_cu, dmi::t2map_lds_bytes(o->nTE, o->nT2, lda, K)));
+    const int64_t slot_doubles = dmi::t2map_slot_doubles(o->nTE, o->nT2) *
+                                 std::max<int64_t>(1, std::min<int64_t>((int64_t)c->num_cu * std::max(per_cu, 1), V));
+    HIPCHECK(c, c->lb.scratch.ensure(sizeof(double) * (size_t)slot_doubles));
+    RunBufs rb{c->grids.ptrs(), c->vox.as<int64_t>(), std::max<int64_t>(V, 1), c->lb.counter.as<unsigned>(),
+               c->lb.counters.as<unsigned long long>(), c->lb.status.as<int>(), c->lb.scratch.as<double>(), slot_doubles};
+    return run_t2map_on(c, d_image, o, d_alpha_in, d, rb);
+}
+
+// The allocation-free form: every buffer lives in the device's static scratch. The grids in it are keyed per
+// device (they belong to whichever call last built them there).
+std::mutex g_noalloc_mu;
+dmi_host::GridKey g_noalloc_key[dmi::kNoAllocMaxDevices];
+
+int run_t2map_noalloc(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, const double *d_alpha_in,
+                      const dmi_t2map_out *d) {
+    int rc = dmi_host::validate_outputs(c->err, o, d);
+    if (rc) return rc;
+    if (!c->noalloc) return fail(c, DMI_EHIP, "static scratch address unavailable on this device");
+    const int m = o->nTE, n = o->nT2, K = std::isnan(o->SetFlipAngle) ? o->nRefAngles : 1;
+    if (m > dmi::kMaxEcho || n > dmi::kMaxT2 || K > dmi::kMaxAngles ||
+        (int64_t)dmi::basis_ld(m) * n * K > dmi::kNoAllocBasis)
+        return dmi_host::errorf(c->err, DMI_ECAPACITY,
+                                "nTE = %d, nT2 = %d, nRefAngles = %d exceed the static scratch (at most %d x %d x %d)", m,
+                                n, K, dmi::kMaxEcho, dmi::kMaxT2, dmi::kMaxAngles);
+    double *S = c->noalloc;
+    RunBufs rb{};
+    rb.g.E2 = S + dmi::kNoAllocOffGrid;
+    rb.g.logT2 = rb.g.E2 + 64;
+    rb.g.alphas = rb.g.E2 + 128;
+    rb.g.As = S + dmi::kNoAllocOffBasis;
+    rb.g.dAs = rb.g.As + dmi::kNoAllocBasis;
+    rb.g.AsT = rb.g.As + 2 * dmi::kNoAllocBasis;
+    rb.g.dAsT = rb.g.As + 3 * dmi::kNoAllocBasis;
+    rb.counters = reinterpret_cast<unsigned long long *>(S);       // S[0, 4)
+    rb.status = reinterpret_cast<int *>(S + 4);                     // S[4, 6)
+    rb.counter = reinterpret_cast<unsigned *>(S + 6);               // S[6, 8)
+    rb.vox = reinterpret_cast<int64_t *>(S + dmi::kNoAllocOffVox);
+    rb.vox_cap = dmi::kNoAllocVox;
+    rb.slots = S + dmi::kNoAllocOffSlots;
+    rb.slot_doubles = dmi::kNoAllocSlots;
+    HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        const dmi_host::GridKey k = dmi_host::GridKey::of(o);
+        dmi_host::GridKey &have = g_noalloc_key[c->device];
+        if (!(have == k)) {
+            have = dmi_host::GridKey{};
+            rc = dmi_host::build_grids(c->err, o, rb.g, c->stream);
+            if (rc) return rc;
+            have = k;
+        }
+    }
+    return run_t2map_on(c, d_image, o, d_alpha_in, d, rb);
 }
 
 int run_t2part(dmi_ctx *c, const double *d_dist, const dmi_t2part_opts *o, double *sfr, double *sgm, double *mfr,
@@ -155,6 +234,12 @@ int dmi_create(dmi_ctx **out, int device, uint32_t flags) {
         return dmi_host::errorf(g_create_error, DMI_EHIP, "stream creation failed: %s", hipGetErrorString(e));
     }
     for (auto &ev : c->ev) (void)hipEventCreate(&ev);
+    if ((e = dmi::noalloc_scratch(device, &c->noalloc)) != hipSuccess) {
+        for (auto &ev : c->ev) (void)hipEventDestroy(ev);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return dmi_host::errorf(g_create_error, DMI_EHIP, "static scratch lookup failed: %s", hipGetErrorString(e));
+    }
     *out = c;
     return DMI_OK;
 }
@@ -193,6 +278,17 @@ int dmi_t2map_f64_device(dmi_ctx *c, const double *d_image, const dmi_t2map_opts
     if (rc) return rc;
     if (!d_image) return fail(c, DMI_EINVAL, "image is NULL");
     return run_t2map(c, d_image, o, d_alpha_in, d_out);
+}
+
+int dmi_t2map_f64_device_noalloc(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o,
+                                 const double *d_alpha_in, const dmi_t2map_out *d_out) {
+    if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHECK(c, hipSetDevice(c->device));
+    int rc = dmi_host::validate_t2map(c->err, o);
+    if (rc) return rc;
+    if (!d_image) return fail(c, DMI_EINVAL, "image is NULL");
+    return run_t2map_noalloc(c, d_image, o, d_alpha_in, d_out);
 }
 
 int dmi_t2map_f64(dmi_ctx *c, const double *image, const dmi_t2map_opts *o, const double *alpha_in,
@@ -324,7 +420,7 @@ int dmi_epg_basis_set(dmi_ctx *c, const dmi_t2map_opts *o, double *out) {
     if (rc) return rc;
     const int m = o->nTE, n = o->nT2, K = search ? o->nRefAngles : 1;
     std::vector<double> rm((size_t)m * n * K);
-    HIPCHECK(c, hipMemcpy(rm.data(), search ? c->grids.As.p : c->grids.Afixed.p, sizeof(double) * rm.size(), hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(rm.data(), c->grids.ptrs().As, sizeof(double) * rm.size(), hipMemcpyDeviceToHost));
     // device layout K x m x n (row-major per angle) -> Julia (nTE, nT2, K) column-major
     for (int k = 0; k < K; ++k)
         for (int j = 0; j < n; ++j)

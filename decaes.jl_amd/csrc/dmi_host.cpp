@@ -139,9 +139,8 @@ int validate_outputs(std::string &err, const dmi_t2map_opts *o, const dmi_t2map_
     return DMI_OK;
 }
 
-int Grids::ensure(std::string &err, const dmi_t2map_opts *o, hipStream_t s) {
+int build_grids(std::string &err, const dmi_t2map_opts *o, const GridPtrs &gp, hipStream_t s) {
     const GridKey k = GridKey::of(o);
-    if (k == key) return DMI_OK;
     const bool search = std::isnan(o->SetFlipAngle);
     const int m = k.m, n = k.n, K = k.K;
     std::vector<double> T2(n), e2(n), lg(n), al(K);
@@ -149,44 +148,66 @@ int Grids::ensure(std::string &err, const dmi_t2map_opts *o, hipStream_t s) {
     for (int j = 0; j < n; ++j) { e2[j] = std::exp(-(o->TE / 2) / T2[j]); lg[j] = std::log(T2[j]); }
     if (search) linrange(o->MinRefAngle, 180.0, K, al.data());
     else al[0] = o->SetFlipAngle;
+    DMI_HIP(err, hipMemcpyAsync(gp.E2, e2.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    DMI_HIP(err, hipMemcpyAsync(gp.logT2, lg.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    DMI_HIP(err, hipMemcpyAsync(gp.alphas, al.data(), sizeof(double) * K, hipMemcpyHostToDevice, s));
+    dmi::GridParams g{};
+    g.m = m; g.n = n; g.K = K; g.general = k.general; g.fp32 = k.fp32; g.beta = o->RefConAngle;
+    g.E1 = std::exp(-(o->TE / 2) / o->T1);
+    g.E2 = gp.E2;
+    g.alphas = gp.alphas;
+    // column-major copies: columns of stride basis_ld(m); the pad row of an odd m stays zero
+    const size_t bytesT = sizeof(double) * (size_t)dmi::basis_ld(m) * n * K;
+    DMI_HIP(err, dmi::launch_fill(gp.AsT, bytesT, 0u, s));
+    if (search) DMI_HIP(err, dmi::launch_fill(gp.dAsT, bytesT, 0u, s));
+    g.As = gp.As;
+    g.dAs = search ? gp.dAs : nullptr;
+    g.AsT = gp.AsT;
+    DMI_HIP(err, dmi::launch_grid_basis(g, s));
+    if (search) DMI_HIP(err, dmi::launch_transpose(gp.dAs, gp.dAsT, m, n, K, s));
+    DMI_HIP(err, hipStreamSynchronize(s));
+    return DMI_OK;
+}
+
+int Grids::ensure(std::string &err, const dmi_t2map_opts *o, hipStream_t s) {
+    const GridKey k = GridKey::of(o);
+    if (k == key) return DMI_OK;
+    const bool search = std::isnan(o->SetFlipAngle);
+    const int m = k.m, n = k.n, K = k.K;
     key = GridKey{};  // invalid until the rebuild completes
     DMI_HIP(err, E2.ensure(sizeof(double) * n));
     DMI_HIP(err, logT2.ensure(sizeof(double) * n));
     DMI_HIP(err, alphas.ensure(sizeof(double) * K));
-    DMI_HIP(err, hipMemcpyAsync(E2.p, e2.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
-    DMI_HIP(err, hipMemcpyAsync(logT2.p, lg.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
-    DMI_HIP(err, hipMemcpyAsync(alphas.p, al.data(), sizeof(double) * K, hipMemcpyHostToDevice, s));
-    dmi::GridParams g{};
-    g.m = m; g.n = n; g.K = K; g.general = k.general; g.fp32 = k.fp32; g.beta = o->RefConAngle;
-    g.E1 = std::exp(-(o->TE / 2) / o->T1);
-    g.E2 = E2.as<double>();
-    g.alphas = alphas.as<double>();
     const size_t bytes = sizeof(double) * (size_t)m * n * K;
-    // column-major copies: columns of stride basis_ld(m); the pad row of an odd m stays zero
     const size_t bytesT = sizeof(double) * (size_t)dmi::basis_ld(m) * n * K;
     if (search) {
         DMI_HIP(err, As.ensure(bytes));
         DMI_HIP(err, dAs.ensure(bytes));
         DMI_HIP(err, AsT.ensure(bytesT));
         DMI_HIP(err, dAsT.ensure(bytesT));
-        DMI_HIP(err, hipMemsetAsync(AsT.p, 0, bytesT, s));
-        DMI_HIP(err, hipMemsetAsync(dAsT.p, 0, bytesT, s));
-        g.As = As.as<double>();
-        g.dAs = dAs.as<double>();
-        g.AsT = AsT.as<double>();
     } else {
         DMI_HIP(err, Afixed.ensure(bytes));
         DMI_HIP(err, AfixedT.ensure(bytesT));
-        DMI_HIP(err, hipMemsetAsync(AfixedT.p, 0, bytesT, s));
-        g.As = Afixed.as<double>();
-        g.dAs = nullptr;
-        g.AsT = AfixedT.as<double>();
     }
-    DMI_HIP(err, dmi::launch_grid_basis(g, s));
-    if (search) DMI_HIP(err, dmi::launch_transpose(dAs.as<double>(), dAsT.as<double>(), m, n, K, s));
-    DMI_HIP(err, hipStreamSynchronize(s));
+    int rc = build_grids(err, o, ptrs_for(search), s);
+    if (rc) return rc;
     key = k;
     return DMI_OK;
+}
+
+GridPtrs Grids::ptrs() const { return ptrs_for(std::isnan(key.setflip)); }
+
+GridPtrs Grids::ptrs_for(bool search) const {
+    GridPtrs g;
+    g.E2 = E2.as<double>();
+    g.logT2 = logT2.as<double>();
+    g.alphas = alphas.as<double>();
+    if (search) {
+        g.As = As.as<double>(); g.dAs = dAs.as<double>(); g.AsT = AsT.as<double>(); g.dAsT = dAsT.as<double>();
+    } else {
+        g.As = Afixed.as<double>(); g.AsT = AfixedT.as<double>();
+    }
+    return g;
 }
 
 void Grids::release() {
@@ -198,7 +219,7 @@ void LaunchBufs::release() {
     for (DevBuf *b : {&counter, &counters, &status, &scratch}) b->release();
 }
 
-dmi::T2mapParams t2map_params(const dmi_t2map_opts *o, const Grids &g, const double *d_image, const double *d_alpha,
+dmi::T2mapParams t2map_params(const dmi_t2map_opts *o, const GridPtrs &g, const double *d_image, const double *d_alpha,
                               const dmi_t2map_out *d) {
     const int m = o->nTE, n = o->nT2;
     const bool fixed = !std::isnan(o->SetFlipAngle);
@@ -218,12 +239,12 @@ dmi::T2mapParams t2map_params(const dmi_t2map_opts *o, const Grids &g, const dou
     p.noiselevel = o->NoiseLevel;
     p.setflip = o->SetFlipAngle;
     p.image = d_image;
-    p.E2 = g.E2.as<double>();
-    p.logT2 = g.logT2.as<double>();
-    p.alphas = g.alphas.as<double>();
-    p.AsT = fixed ? nullptr : g.AsT.as<double>();
-    p.dAsT = fixed ? nullptr : g.dAsT.as<double>();
-    p.Afixed = fixed ? g.AfixedT.as<double>() : nullptr;
+    p.E2 = g.E2;
+    p.logT2 = g.logT2;
+    p.alphas = g.alphas;
+    p.AsT = fixed ? nullptr : g.AsT;
+    p.dAsT = fixed ? nullptr : g.dAsT;
+    p.Afixed = fixed ? g.AsT : nullptr;
     p.alpha_in = d_alpha;
     p.dist = d->dist; p.gdn = d->gdn; p.ggm = d->ggm; p.gva = d->gva; p.fnr = d->fnr; p.snr = d->snr; p.alpha = d->alpha;
     p.resnorm = o->SaveResidualNorm ? d->resnorm : nullptr;
@@ -232,6 +253,30 @@ dmi::T2mapParams t2map_params(const dmi_t2map_opts *o, const Grids &g, const dou
     p.chi2f = o->SaveRegParam ? d->chi2factor : nullptr;
     p.decaybasis = (o->SaveNNLSBasis && !fixed) ? d->decaybasis : nullptr;
     return p;
+}
+
+int t2map_launch_raw(std::string &err, dmi::T2mapParams p, const int64_t *d_vox, int64_t nvox, unsigned *counter,
+                     unsigned long long *counters, int *status, double *slots, int64_t slot_doubles, int num_cu,
+                     hipStream_t s) {
+    if (nvox <= 0) return DMI_OK;
+    const size_t lds = dmi::t2map_lds_bytes(p.m, p.n, p.lda, p.K);
+    int per_cu = 0;
+    DMI_HIP(err, dmi::t2map_occupancy(p.reg, &per_cu, lds));
+    if (per_cu < 1) return errorf(err, DMI_EUNSUPPORTED, "voxel kernel does not fit on a CU (LDS %zu bytes)", lds);
+    p.slot_stride = dmi::t2map_slot_doubles(p.m, p.n);
+    const int64_t fit = slot_doubles / p.slot_stride;
+    if (fit < 1) return errorf(err, DMI_ECAPACITY, "scratch of %lld doubles holds no voxel slot of %lld doubles",
+                               (long long)slot_doubles, (long long)p.slot_stride);
+    const int grid = (int)std::min<int64_t>(std::min<int64_t>((int64_t)num_cu * per_cu, nvox), fit);
+    p.scratch = slots;
+    p.vox = d_vox;
+    p.nvox = nvox;
+    p.counter = counter;
+    p.counters = counters;
+    p.status = status;
+    DMI_HIP(err, dmi::launch_fill(counter, sizeof(unsigned), 0u, s));
+    DMI_HIP(err, dmi::launch_t2map(p, grid, lds, s));
+    return DMI_OK;
 }
 
 int t2map_launch(std::string &err, dmi::T2mapParams p, const int64_t *d_vox, int64_t nvox, LaunchBufs &lb,
@@ -244,18 +289,11 @@ int t2map_launch(std::string &err, dmi::T2mapParams p, const int64_t *d_vox, int
     int per_cu = 0;
     DMI_HIP(err, dmi::t2map_occupancy(p.reg, &per_cu, lds));
     if (per_cu < 1) return errorf(err, DMI_EUNSUPPORTED, "voxel kernel does not fit on a CU (LDS %zu bytes)", lds);
-    const int grid = (int)std::min<int64_t>((int64_t)num_cu * per_cu, nvox);
-    p.slot_stride = dmi::t2map_slot_doubles(p.m, p.n);
-    DMI_HIP(err, lb.scratch.ensure(sizeof(double) * (size_t)p.slot_stride * grid));
-    p.scratch = lb.scratch.as<double>();
-    p.vox = d_vox;
-    p.nvox = nvox;
-    p.counter = lb.counter.as<unsigned>();
-    p.counters = lb.counters.as<unsigned long long>();
-    p.status = lb.status.as<int>();
-    DMI_HIP(err, hipMemsetAsync(lb.counter.p, 0, sizeof(unsigned), s));
-    DMI_HIP(err, dmi::launch_t2map(p, grid, lds, s));
-    return DMI_OK;
+    const int64_t grid = std::min<int64_t>((int64_t)num_cu * per_cu, nvox);
+    const int64_t slot_doubles = dmi::t2map_slot_doubles(p.m, p.n) * grid;
+    DMI_HIP(err, lb.scratch.ensure(sizeof(double) * (size_t)slot_doubles));
+    return t2map_launch_raw(err, p, d_vox, nvox, lb.counter.as<unsigned>(), lb.counters.as<unsigned long long>(),
+                            lb.status.as<int>(), lb.scratch.as<double>(), slot_doubles, num_cu, s);
 }
 
 int t2part_prepare(std::string &err, const dmi_t2part_opts *o, std::vector<double> &lg, std::vector<double> &wts,

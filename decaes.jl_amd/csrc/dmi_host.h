@@ -39,12 +39,25 @@ struct GridKey {
     static GridKey of(const dmi_t2map_opts *o);
 };
 
+// Device arrays of one option set's grids. With SetFlipAngle, As / AsT hold the fixed basis (row-major and
+// column-major) and dAs / dAsT are unused.
+struct GridPtrs {
+    double *E2 = nullptr, *logT2 = nullptr, *alphas = nullptr;
+    double *As = nullptr, *dAs = nullptr, *AsT = nullptr, *dAsT = nullptr;
+};
+
+// Fills the grids of `o` into `g` (T2 grid, flip angles, EPG bases and their alpha-derivatives); allocates nothing,
+// asynchronous on `s` except for the small host-to-device copies of the grids.
+int build_grids(std::string &err, const dmi_t2map_opts *o, const GridPtrs &g, hipStream_t s);
+
 // T2 grid, flip-angle grid and the EPG basis sets over them, on one device.
 struct Grids {
     GridKey key;
     DevBuf E2, logT2, alphas, As, dAs, Afixed, AsT, dAsT, AfixedT;
     // (Re)builds the bases for `o` unless they are already current; synchronizes `s`.
     int ensure(std::string &err, const dmi_t2map_opts *o, hipStream_t s);
+    GridPtrs ptrs() const;
+    GridPtrs ptrs_for(bool search) const;
     void release();
 };
 
@@ -65,13 +78,18 @@ int validate_outputs(std::string &err, const dmi_t2map_opts *o, const dmi_t2map_
 
 // Kernel parameters for one request whose image, B1 map and outputs are device arrays; the voxel list and
 // count are set per launch.
-dmi::T2mapParams t2map_params(const dmi_t2map_opts *o, const Grids &g, const double *d_image, const double *d_alpha,
+dmi::T2mapParams t2map_params(const dmi_t2map_opts *o, const GridPtrs &g, const double *d_image, const double *d_alpha,
                               const dmi_t2map_out *d);
 
 // Launches the voxel kernel on voxels d_vox[0, nvox) of the request in `p` (asynchronous on `s`). Zeroes the
 // work counter; tallies and status accumulate into lb.counters / lb.status until the caller clears them.
 int t2map_launch(std::string &err, dmi::T2mapParams p, const int64_t *d_vox, int64_t nvox, LaunchBufs &lb,
                  int num_cu, hipStream_t s);
+// The same launch on caller-provided device memory: the work counter (zeroed here by a fill kernel), the tallies and
+// status words, and `slot_doubles` doubles of per-workgroup slots, which cap the persistent grid.
+int t2map_launch_raw(std::string &err, dmi::T2mapParams p, const int64_t *d_vox, int64_t nvox, unsigned *counter,
+                     unsigned long long *counters, int *status, double *slots, int64_t slot_doubles, int num_cu,
+                     hipStream_t s);
 
 // T2partSEcorr window ranges, log-T2 values and sigmoid weights (src/T2partSEcorr.jl:143-164).
 int t2part_prepare(std::string &err, const dmi_t2part_opts *o, std::vector<double> &lg, std::vector<double> &wts,

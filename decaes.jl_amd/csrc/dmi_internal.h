@@ -17,6 +17,21 @@ constexpr int kLcurveCap = 128;   // L-curve cache capacity per voxel (reference
 // up to even, so every column starts 16-byte aligned and is copied into LDS with 16-byte global-to-LDS loads.
 __host__ __device__ inline int basis_ld(int m) { return (m + 1) & ~1; }
 
+// Static scratch of the allocation-free entry point, in doubles: a header (launch counters), the T2 and angle grids,
+// four grid-basis arrays at the device limits (row-major As, dAs; column-major AsT, dAsT), a voxel-list chunk
+// (int64) and the per-workgroup slots of the voxel kernel.
+constexpr int64_t kNoAllocHeader = 32;
+constexpr int64_t kNoAllocGrid = 3 * 64;
+constexpr int64_t kNoAllocBasis = (int64_t)kMaxAngles * kMaxT2 * kMaxEcho;   // kMaxEcho is even: basis_ld = m
+constexpr int64_t kNoAllocVox = 262144;                                      // voxels selected per chunk
+constexpr int64_t kNoAllocSlots = 5242880;                                   // 40 MiB of slots
+constexpr int64_t kNoAllocOffGrid = kNoAllocHeader;
+constexpr int64_t kNoAllocOffBasis = kNoAllocOffGrid + kNoAllocGrid;
+constexpr int64_t kNoAllocOffVox = kNoAllocOffBasis + 4 * kNoAllocBasis;
+constexpr int64_t kNoAllocOffSlots = kNoAllocOffVox + kNoAllocVox;
+constexpr int64_t kNoAllocDoubles = kNoAllocOffSlots + kNoAllocSlots;
+constexpr int kNoAllocMaxDevices = 64;
+
 enum StatusBits : int {
     kStatusLcurveOverflow = 1,
     kStatusNnlsMaxIter = 2,
@@ -78,8 +93,13 @@ int64_t t2map_slot_doubles(int m, int n);
 
 hipError_t launch_grid_basis(const GridParams &p, hipStream_t s);
 hipError_t launch_transpose(const double *src, double *dst, int m, int n, int K, hipStream_t s);
-hipError_t launch_select(const double *image, int64_t V, double threshold, int64_t *vox, unsigned int *count,
+// voxels I in [v0, v1) with image[I] > thr (image[I] is echo 1 of voxel I)
+hipError_t launch_select(const double *image, int64_t v0, int64_t v1, double thr, int64_t *vox, unsigned int *count,
                          hipStream_t s);
+// fills bytes / 4 32-bit words with v
+hipError_t launch_fill(void *p, size_t bytes, unsigned int v, hipStream_t s);
+// This device's address of the static scratch (looked up once per device with the device made current).
+hipError_t noalloc_scratch(int device, double **ptr);
 hipError_t launch_t2map(const T2mapParams &p, int grid, size_t lds, hipStream_t s);
 hipError_t launch_t2part(const T2partParams &p, hipStream_t s);
 // Float32 <-> Float64 element-wise conversion (Float32 entry points)

@@ -13,6 +13,7 @@
 // Arithmetic follows the reference statement by statement, with fma() where DECAES writes muladd or sits
 // under @muladd (src/NNLS.jl:47, src/EPGdecaycurve.jl:858-1024); built with -ffp-contract=off.
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include <type_traits>
 
 #include <cfloat>
@@ -515,10 +516,24 @@ __device__ unsigned long long g_eig_steps[32];
 // Voxel selection: indices with image[I, 1] > Threshold (src/T2mapSEcorr.jl:177). Order is irrelevant:
 // voxels are independent and results are written at their own index.
 // ------------------------------------------------------------------------------------------------
-__global__ void select_kernel(const double *image, int64_t V, double thr, int64_t *vox, unsigned int *count) {
-    for (int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I < V; I += (int64_t)gridDim.x * blockDim.x)
+__global__ void select_kernel(const double *image, int64_t v0, int64_t v1, double thr, int64_t *vox, unsigned int *count) {
+    for (int64_t I = v0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I < v1; I += (int64_t)gridDim.x * blockDim.x)
         if (image[I] > thr) vox[atomicAdd(count, 1u)] = I;
 }
+
+// 32-bit word fill, so counters and padding are cleared by a kernel node rather than a memset
+__global__ void fill_kernel(unsigned int *p, int64_t nwords, unsigned int v) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nwords; t += (int64_t)gridDim.x * blockDim.x)
+        p[t] = v;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Static scratch of the allocation-free entry point (dmi_t2map_f64_device_noalloc): grid bases at the device
+// limits, a voxel-list chunk, the per-workgroup slots of the voxel kernel and the launch counters, in one array
+// each device holds for the life of the process. Host code never passes `scratch` itself to a kernel: it takes
+// the device's own address of the array once with hipGetSymbolAddress (noalloc_scratch below).
+// ------------------------------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
 
 // ------------------------------------------------------------------------------------------------
 // Per-voxel wave state
@@ -2339,10 +2354,11 @@ hipError_t launch_narrow(const double *src, float *dst, int64_t n, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_select(const double *image, int64_t V, double thr, int64_t *vox, unsigned int *count, hipStream_t s) {
-    int blocks = (int)std::min<int64_t>((V + 255) / 256, 4096);
+hipError_t launch_select(const double *image, int64_t v0, int64_t v1, double thr, int64_t *vox, unsigned int *count,
+                         hipStream_t s) {
+    int blocks = (int)std::min<int64_t>((v1 - v0 + 255) / 256, 4096);
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(select_kernel, dim3(blocks), dim3(256), 0, s, image, V, thr, vox, count);
+    hipLaunchKernelGGL(select_kernel, dim3(blocks), dim3(256), 0, s, image, v0, v1, thr, vox, count);
     return hipGetLastError();
 }
 
@@ -2364,6 +2380,34 @@ hipError_t launch_t2map(const T2mapParams &p, int grid, size_t lds, hipStream_t 
 
 hipError_t t2map_occupancy(int reg, int *blocks_per_cu, size_t lds) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, t2map_kernel_for(reg), 64, lds);
+}
+
+hipError_t launch_fill(void *p, size_t bytes, unsigned int v, hipStream_t s) {
+    const int64_t nw = (int64_t)(bytes / 4);
+    if (nw <= 0) return hipSuccess;
+    const int blocks = (int)std::min<int64_t>((nw + 255) / 256, 2048);
+    hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, s, (unsigned int *)p, nw, v);
+    return hipGetLastError();
+}
+
+hipError_t noalloc_scratch(int device, double **ptr) {
+    static std::mutex mu;
+    static double *addr[kNoAllocMaxDevices] = {};
+    if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+        void *a = nullptr;
+        e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        addr[device] = static_cast<double *>(a);
+    }
+    *ptr = addr[device];
+    return hipSuccess;
 }
 
 hipError_t launch_t2part(const T2partParams &p, hipStream_t s) {

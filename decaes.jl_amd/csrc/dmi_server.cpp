@@ -234,7 +234,7 @@ int bind_and_select(dmi_server *S, StreamState &st, Request *r) {
     if (rc) return rc;
     const bool fixed = !std::isnan(r->o.SetFlipAngle);
     if (fixed && r->o.SaveNNLSBasis)
-        DMI_HIP(e, hipMemcpy2DAsync(r->d_out[11], sizeof(double) * r->o.nTE, st.grids.AfixedT.p,
+        DMI_HIP(e, hipMemcpy2DAsync(r->d_out[11], sizeof(double) * r->o.nTE, st.grids.ptrs().AsT,
                                     sizeof(double) * dmi::basis_ld(r->o.nTE), sizeof(double) * r->o.nTE, r->o.nT2,
                                     hipMemcpyDeviceToDevice, st.s));
     // scratch for the largest grid any regularization method can launch with these sizes, so it never grows
@@ -252,19 +252,19 @@ int bind_and_select(dmi_server *S, StreamState &st, Request *r) {
     DMI_HIP(e, st.lb.counter.ensure(sizeof(unsigned) * 4));
     DMI_HIP(e, st.lb.counters.ensure(sizeof(unsigned long long) * 4));
     DMI_HIP(e, st.lb.status.ensure(sizeof(int) * 4));
-    DMI_HIP(e, hipMemsetAsync(st.lb.counter.p, 0, sizeof(unsigned), st.s));
-    DMI_HIP(e, dmi::launch_select(r->d_image, r->V, r->o.Threshold, r->d_vox, st.lb.counter.as<unsigned>(), st.s));
+    DMI_HIP(e, dmi::launch_fill(st.lb.counter.p, sizeof(unsigned), 0u, st.s));
+    DMI_HIP(e, dmi::launch_select(r->d_image, 0, r->V, r->o.Threshold, r->d_vox, st.lb.counter.as<unsigned>(), st.s));
     DMI_HIP(e, hipMemcpyAsync(r->h_nvox, st.lb.counter.p, sizeof(unsigned), hipMemcpyDeviceToHost, st.s));
     DMI_HIP(e, hipEventRecord(r->ev, st.s));
     dmi_t2map_out d{r->d_out[0], r->d_out[1], r->d_out[2], r->d_out[3], r->d_out[4], r->d_out[5],
                     r->d_out[6], r->d_out[7], r->d_out[8], r->d_out[9], r->d_out[10], r->d_out[11]};
-    r->params = dmi_host::t2map_params(&r->o, st.grids, r->d_image, r->d_alpha, &d);
+    r->params = dmi_host::t2map_params(&r->o, st.grids.ptrs(), r->d_image, r->d_alpha, &d);
     return DMI_OK;
 }
 
 int launch_chunk(dmi_server *S, StreamState &st, Request *r, int64_t count, int slot) {
     std::string &e = r->err;
-    DMI_HIP(e, hipMemsetAsync(st.lb.status.p, 0, sizeof(int), st.s));
+    DMI_HIP(e, dmi::launch_fill(st.lb.status.p, sizeof(int), 0u, st.s));
     int rc = dmi_host::t2map_launch(e, r->params, r->d_vox + r->launched, count, st.lb, S->num_cu, st.s);
     if (rc) return rc;
     DMI_HIP(e, hipMemcpyAsync(&st.h_status[slot], st.lb.status.p, sizeof(int), hipMemcpyDeviceToHost, st.s));

@@ -29,7 +29,7 @@ _dp = C.POINTER(C.c_double)
 EXPORTS = (
     "dmi_create", "dmi_destroy", "dmi_last_error", "dmi_t2map_f64", "dmi_t2map_f64_device", "dmi_epg_basis_set",
     "dmi_t2part_f64", "dmi_t2part_f64_device", "dmi_last_stats", "dmi_synth_mse",
-    "dmi_t2map_f32", "dmi_t2part_f32",
+    "dmi_t2map_f32", "dmi_t2part_f32", "dmi_t2map_f64_device_noalloc",
     "dmi_server_create", "dmi_server_destroy", "dmi_server_submit", "dmi_server_wait", "dmi_server_poll",
     "dmi_server_get_stats", "dmi_server_last_error", "dmi_server_request_error",
 )
@@ -107,6 +107,7 @@ def load() -> C.CDLL:
         L.dmi_last_error.restype = C.c_char_p
         L.dmi_t2map_f64.argtypes = [vp, _dp, C.POINTER(T2mapOptsC), _dp, C.POINTER(T2mapOutC)]
         L.dmi_t2map_f64_device.argtypes = [vp, vp, C.POINTER(T2mapOptsC), vp, C.POINTER(T2mapOutC)]
+        L.dmi_t2map_f64_device_noalloc.argtypes = [vp, vp, C.POINTER(T2mapOptsC), vp, C.POINTER(T2mapOutC)]
         L.dmi_t2map_f32.argtypes = [vp, _fp, C.POINTER(T2mapOptsC), _fp, C.POINTER(T2mapOutF32C)]
         L.dmi_t2part_f32.argtypes = [vp, _fp, C.POINTER(T2partOptsC), _fp, _fp, _fp, _fp]
         L.dmi_epg_basis_set.argtypes = [vp, C.POINTER(T2mapOptsC), _dp]

@@ -42,13 +42,17 @@ def _dptr(t):
     return C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_double))
 
 
-def t2map_device(image, opts: T2mapOptions, out: dict, alpha=None, ctx: Optional[_lib.Context] = None) -> dict:
-    """dmi_t2map_f64_device on torch tensors; returns the call's device counters (dmi_last_stats)."""
+def t2map_device(image, opts: T2mapOptions, out: dict, alpha=None, ctx: Optional[_lib.Context] = None,
+                 noalloc: bool = False) -> dict:
+    """dmi_t2map_f64_device on torch tensors (dmi_t2map_f64_device_noalloc with noalloc=True: no device allocation
+    in the call); returns the call's device counters (dmi_last_stats)."""
     ctx = ctx or _lib.context(image.device.index or 0)
     oc = t2map_opts_c(opts)
     oo = _lib.T2mapOutC(_dptr(out["dist"]), *(_dptr(out.get(k)) for k in (
         "gdn", "ggm", "gva", "fnr", "snr", "alpha", *_OPTIONAL)))
-    rc = _lib.load().dmi_t2map_f64_device(ctx.handle, C.c_void_p(image.data_ptr()), C.byref(oc),
+    L = _lib.load()
+    fn = L.dmi_t2map_f64_device_noalloc if noalloc else L.dmi_t2map_f64_device
+    rc = fn(ctx.handle, C.c_void_p(image.data_ptr()), C.byref(oc),
                                           C.c_void_p(alpha.data_ptr()) if alpha is not None else None, C.byref(oo))
     _lib.check(rc, ctx.handle)
     return ctx.stats()

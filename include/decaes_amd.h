@@ -106,6 +106,16 @@ int dmi_t2map_f64(dmi_ctx *ctx, const double *image, const dmi_t2map_opts *opts,
 int dmi_t2map_f64_device(dmi_ctx *ctx, const double *d_image, const dmi_t2map_opts *opts, const double *d_alpha_in,
                          const dmi_t2map_out *d_out);
 
+/* dmi_t2map_f64_device without any device allocation. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Intermediate buffers (grid bases, voxel list, per-workgroup slots, counters) live in a static device array of
+ * about 60 MB that each device holds for the life of the process; the caller passes inputs and outputs as for
+ * dmi_t2map_f64_device. Limits: nTE <= 124, nT2 <= 64, nRefAngles <= 64 (DMI_ECAPACITY beyond); the volume is
+ * processed in chunks of 262144 voxels, so its size is not limited. Every call on one device shares that array:
+ * calls that overlap on one device (from several contexts or threads) race and must be serialized by the caller. */
+int dmi_t2map_f64_device_noalloc(dmi_ctx *ctx, const double *d_image, const dmi_t2map_opts *opts,
+                                 const double *d_alpha_in, const dmi_t2map_out *d_out);
+
 /* Float32 arrays: the reference is generic in the element type (T2mapSEcorr(image::Array{T,4}, opts::
  * T2mapOptions{T}), src/T2mapSEcorr.jl:148-149, and T2partSEcorr(dist::Array{T,4}, ...), src/T2partSEcorr.jl:40).
  * Here a Float32 image is widened to FP64 on the device, the FP64 path runs, and every output is rounded to

@@ -4,6 +4,14 @@ from pathlib import Path
 
 import pytest
 
+# PyTorch (device buffers in the device-array tests) ships its own HIP runtime and exports it process-wide. When both
+# are in one process it must be loaded before libdecaes_amd.so, whose HIP calls then bind to that same runtime; a
+# second runtime initialized after torch's finds no device.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = Path(__file__).resolve().parents[1]
 for p in (ROOT / "decaes.jl_amd", ROOT / "tests", ROOT):
     if str(p) not in sys.path:
