@@ -11,7 +11,7 @@ constexpr int kMaxT2 = 64;        // one lane per T2 column
 constexpr int kMaxEcho = 124;     // lane-per-state EPG window (ETL/2 + 2 states)
 constexpr int kMaxAngles = 64;    // one lane per flip-angle grid point
 constexpr int kTikhCache = 8;     // NNLSTikhonovRegProblemCache size (src/lsqnonneg.jl:396)
-constexpr int kLcurveCap = 128;   // L-curve cache capacity per voxel (reference starts at 64 and grows)
+constexpr int kLcurveCap = 32;    // L-curve cache entries per voxel: at most 28 are used (4 + 24 bracket moves)
 
 // Column stride of the column-major bases in device memory (grid bases, fixed basis, the voxel's basis): m rounded
 // up to even, so every column starts 16-byte aligned and is copied into LDS with 16-byte global-to-LDS loads.
@@ -33,7 +33,6 @@ constexpr int64_t kNoAllocDoubles = kNoAllocOffSlots + kNoAllocSlots;
 constexpr int kNoAllocMaxDevices = 64;
 
 enum StatusBits : int {
-    kStatusLcurveOverflow = 1,
     kStatusNnlsMaxIter = 2,
 };
 

@@ -551,8 +551,9 @@ struct Wave {
     gdbl *A0s, *lc;
     cgdbl *A0;   // basis used by the regularization solves (A0s or the fixed basis), column-major, stride basis_ld(m)
     int ccur;
-    unsigned long long flops, flops32, solves;
+    unsigned long long flops, flops32, solves, nfailed;
     int status;
+    bool failed;  // this voxel gets no outputs (a per-voxel cache overflowed)
 
     __device__ __forceinline__ ldbl &Aat(int i, int j) { return A[j * lda + i]; }
 
@@ -1668,23 +1669,26 @@ __device__ __forceinline__ void reg_gcv(Wave &W, double &mu_out, double &chi2_ou
 // lsqnonneg_lcurve! (:812-840) with lcurve_corner (:872-972) and the isapprox-keyed GrowableCache /
 // CachedFunction / mapfindmax semantics of src/utils.jl:140-254. Caches live in global scratch; scans are
 // lane-parallel reductions that reproduce the reference's sequential first-match / first-max rules.
+//
+// Sizes. Every move (move_left :933-939, move_right :941-946) shrinks the bracket from x4 - x1 to x3 - x1 = x4 - x2,
+// a factor 1/phi, and the loop stops once the bracket is narrower than xtol: from [-8, 2] with xtol = 1e-4 that is
+// at most ceil(log(1e5) / log(phi)) = 24 moves, each evaluating one new point, so the function and point caches hold
+// at most 4 + 24 = 28 entries (kLcurveCap = 32). The same shrinking makes the backtracking of :892-900 keep the
+// current state: it may only switch to a cached state no wider than the current one, and every earlier state is
+// wider, so the state cache is not kept. A voxel whose caches would still overflow gets no outputs (they stay NaN)
+// and is counted; the rest of the volume is unaffected.
 struct LCurve {
     Wave &W;
     gdbl *fcx, *fcP0, *fcP1;
     gdbl *pcx, *pcP0, *pcP1, *pcC;
-    gdbl *sx[4], *sP[4][2];
-    int nfc, npc, nsc;
+    int nfc, npc;
     bool overflow;
 
-    __device__ __forceinline__ LCurve(Wave &w) : W(w), nfc(0), npc(0), nsc(0), overflow(false) {
+    __device__ __forceinline__ LCurve(Wave &w) : W(w), nfc(0), npc(0), overflow(false) {
         gdbl *p = w.lc;
         fcx = p; fcP0 = p + kLcurveCap; fcP1 = p + 2 * kLcurveCap;
         p += 3 * kLcurveCap;
         pcx = p; pcP0 = p + kLcurveCap; pcP1 = p + 2 * kLcurveCap; pcC = p + 3 * kLcurveCap;
-        p += 4 * kLcurveCap;
-        for (int i = 0; i < 4; ++i) { sx[i] = p; p += kLcurveCap; }
-        for (int i = 0; i < 4; ++i)
-            for (int c = 0; c < 2; ++c) { sP[i][c] = p; p += kLcurveCap; }
     }
     __device__ __forceinline__ int find(const gdbl *keys, int cnt, double x) {
         int first = INT_MAX;
@@ -1804,33 +1808,7 @@ struct LCurve {
         int iter = 0;
         while (!overflow && !(fabs(xs[3] - xs[0]) < xtol || norm2(Ps[0][0] - Ps[3][0], Ps[0][1] - Ps[3][1]) < Ptol)) {
             iter += 1;
-            // backtracking: the last state whose x2 or x3 equals the best point and whose width is the
-            // smallest width not exceeding the current one (same state the sequential scan ends on)
-            double xb = rfl(pcx[pc_argmax()]);
-            double w0 = fabs(xs[3] - xs[0]);
-            double bw = INFINITY;
-            int bk = -1;
-            for (int base = 0; base < nsc; base += kWave) {
-                int k = base + W.L;
-                if (k < nsc) {
-                    double s1 = sx[1][k], s2 = sx[2][k], wk = fabs(sx[3][k] - sx[0][k]);
-                    if ((s1 == xb || s2 == xb) && wk <= w0 && (bk < 0 || wk < bw || (wk == bw && k > bk))) { bw = wk; bk = k; }
-                }
-            }
-#pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) {
-                double ow = __shfl_xor(bw, o, kWave);
-                int ok = __shfl_xor(bk, o, kWave);
-                if (ok >= 0 && (bk < 0 || ow < bw || (ow == bw && ok > bk))) { bw = ow; bk = ok; }
-            }
-            bk = rfl(bk);
-            if (bk >= 0) {
-                for (int i = 0; i < 4; ++i) {
-                    xs[i] = rfl(sx[i][bk]);
-                    Ps[i][0] = rfl(sP[i][0][bk]);
-                    Ps[i][1] = rfl(sP[i][1][bk]);
-                }
-            }
+            // backtracking keeps the current state (see above)
             int i2 = find(pcx, npc, xs[1]), i3 = find(pcx, npc, xs[2]);
             double C2 = rfl(pcC[i2]), C3 = rfl(pcC[i3]);
             if (C2 > C3) {
@@ -1853,12 +1831,6 @@ struct LCurve {
                 for (int i = 0; i < 4; ++i) { xs[i] = nx[i]; Ps[i][0] = nP[i][0]; Ps[i][1] = nP[i][1]; }
             }
             update_curvature(xs, Ps, Ptl, Pbr, Ctol);
-            if (nsc >= kLcurveCap) { overflow = true; break; }
-            if (W.L == 0) {
-                for (int i = 0; i < 4; ++i) { sx[i][nsc] = xs[i]; sP[i][0][nsc] = Ps[i][0]; sP[i][1][nsc] = Ps[i][1]; }
-            }
-            nsc++;
-            __syncthreads();
         }
         return rfl(pcx[pc_argmax()]);
     }
@@ -1868,7 +1840,7 @@ __device__ __forceinline__ void reg_lcurve(Wave &W, double &mu_out, double &chi2
     W.cache_reset();
     LCurve lc(W);
     double logmu = lc.corner(-8.0, 2.0, 1e-4, 1e-4, 1e-4);
-    if (lc.overflow) W.status |= kStatusLcurveOverflow;
+    if (lc.overflow) W.failed = true;
     double mu_final = exp(logmu);
     W.cache_solve(mu_final);
     W.take_cur_solution();
@@ -2138,6 +2110,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     W.flops = 0;
     W.flops32 = 0;
     W.solves = 0;
+    W.nfailed = 0;
     W.status = 0;
     W.ccur = 0;
     DMI_PHASE_BEGIN();
@@ -2184,6 +2157,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
         }
         // T2_distribution! (src/T2mapSEcorr.jl:475-505)
         double muv = 0.0, chi2v = 1.0;
+        W.failed = false;
         DMI_PT0();
         if constexpr (REG == 0) W.plain_solve_to_xf();
         else if constexpr (REG == 1) reg_lcurve(W, muv, chi2v);
@@ -2195,6 +2169,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
             reg_mdp(W, delta, muv, chi2v);
         }
         DMI_PT1(kPhRegTotal);
+        if (W.failed) {  // outputs stay as the caller filled them (NaN)
+            W.nfailed += 1;
+            __syncthreads();
+            continue;
+        }
         // save_results! (:512-591)
         double xs = (L < n) ? W.xf[L] * max_signal : 0.0;
         if (L < n) W.xf[L] = xs;
@@ -2246,6 +2225,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
         atomicAdd(&p.counters[0], W.flops);
         atomicAdd(&p.counters[1], W.solves);
         atomicAdd(&p.counters[2], W.flops32);
+        if (W.nfailed) atomicAdd(&p.counters[3], W.nfailed);
         if (W.status) atomicOr(p.status, W.status);
     }
 }
@@ -2312,7 +2292,7 @@ size_t t2map_lds_bytes(int m, int n, int lda, int K) {
 }
 
 int64_t t2map_slot_doubles(int m, int n) {
-    int64_t s = (int64_t)basis_ld(m) * n + 19 * (int64_t)kLcurveCap;
+    int64_t s = (int64_t)basis_ld(m) * n + 7 * (int64_t)kLcurveCap;  // basis, function cache (3), point cache (4)
     return (s + 31) / 32 * 32;
 }
 

@@ -82,6 +82,8 @@ struct StreamState {
 };
 
 constexpr int kMaxInflight = 2;
+constexpr size_t kLatencyWindow = 65536;  // latencies of the most recent completed requests kept for p50/p99
+constexpr size_t kErrorWindow = 256;      // error texts of waited-for failed requests kept for request_error
 
 }  // namespace
 
@@ -92,12 +94,15 @@ struct dmi_server {
     std::vector<StreamState> streams;
     std::mutex mu;
     std::condition_variable cv_sched, cv_copy, cv_done;
-    std::map<int64_t, std::unique_ptr<Request>> reqs;
+    std::map<int64_t, std::shared_ptr<Request>> reqs;  // submitted and not yet returned by dmi_server_wait
+    std::vector<Request *> active;    // admitted and not done (what the scheduler scans)
+    std::deque<std::pair<int64_t, std::string>> waited_errors;  // recent failures already returned by wait
     std::deque<Request *> queued;     // waiting for admission
     std::deque<Request *> copy_jobs;  // uploads and downloads
     int64_t next_id = 1, used_bytes = 0, live = 0;
     int64_t submitted = 0, completed = 0, failed = 0, voxels = 0;
-    std::vector<double> latencies;
+    std::vector<double> latencies;    // ring of the last kLatencyWindow latencies
+    size_t lat_next = 0;
     double busy_ms = 0;
     Clock::time_point busy_since;
     bool stop = false;
@@ -116,10 +121,14 @@ void mark_done(dmi_server *S, Request *r) {
     if (r->status == DMI_OK) {
         ++S->completed;
         S->voxels += std::max<int64_t>(r->nvox, 0);
-        S->latencies.push_back(ms_between(r->t_submit, r->t_done));
+        const double lat = ms_between(r->t_submit, r->t_done);
+        if (S->latencies.size() < kLatencyWindow) S->latencies.push_back(lat);
+        else S->latencies[S->lat_next] = lat;
+        S->lat_next = (S->lat_next + 1) % kLatencyWindow;
     } else {
         ++S->failed;
     }
+    S->active.erase(std::remove(S->active.begin(), S->active.end(), r), S->active.end());
     S->cv_done.notify_all();
     S->cv_sched.notify_all();
     S->cv_copy.notify_all();
@@ -299,6 +308,7 @@ void sched_loop(dmi_server *S) {
                 Request *r = S->queued.front();
                 if (r->bytes > budget) {
                     S->queued.pop_front();
+                    S->active.push_back(r);
                     fail_request(r, DMI_ECAPACITY, "request needs " + std::to_string(r->bytes) +
                                                        " bytes of device memory, more than the budget of " + std::to_string(budget));
                     ++S->live;
@@ -309,6 +319,7 @@ void sched_loop(dmi_server *S) {
                 }
                 if (S->used_bytes + r->bytes > budget) break;
                 S->queued.pop_front();
+                S->active.push_back(r);
                 S->used_bytes += r->bytes;
                 r->t_admit = Clock::now();
                 if (S->live++ == 0) S->busy_since = r->t_admit;
@@ -317,8 +328,7 @@ void sched_loop(dmi_server *S) {
                 S->cv_copy.notify_all();
                 progress = true;
             }
-            for (auto &kv : S->reqs) {
-                Request *r = kv.second.get();
+            for (Request *r : S->active) {
                 if (r->st == St::Uploaded) ready_to_bind.push_back(r);
                 else if (r->st == St::Selecting) selecting.push_back(r);
                 else if (r->st == St::Finishing) finishing.push_back(r);
@@ -373,11 +383,6 @@ void sched_loop(dmi_server *S) {
                 (void)hipEventDestroy(c.ev);
                 st.slot_busy[c.slot] = false;
                 Request *r = c.r;
-                if (st.h_status[c.slot] & dmi::kStatusLcurveOverflow) {
-                    std::lock_guard<std::mutex> lk(S->mu);
-                    fail_request(r, DMI_ECAPACITY, "L-curve cache exceeded " + std::to_string(dmi::kLcurveCap) +
-                                                       " entries for at least one voxel");
-                }
                 r->finished += c.count;
                 if (r->finished == r->nvox) {
                     int rc = launch_finish(st, r);
@@ -518,7 +523,7 @@ int dmi_server_submit(dmi_server *S, const double *image, const dmi_t2map_opts *
     int rc = dmi_host::validate_t2map(e, o);
     if (!rc) rc = dmi_host::validate_outputs(e, o, out);
     if (!rc && !image) rc = dmi_host::errorf(e, DMI_EINVAL, "image is NULL");
-    auto r = std::make_unique<Request>();
+    auto r = std::make_shared<Request>();
     if (!rc && popts) {
         if (!sfr || !sgm || !mfr || !mgm) rc = dmi_host::errorf(e, DMI_EINVAL, "T2part outputs sfr, sgm, mfr and mgm are required");
         else if (popts->nT2 != o->nT2 || popts->MatrixSize[0] != o->MatrixSize[0] || popts->MatrixSize[1] != o->MatrixSize[1] ||
@@ -569,7 +574,7 @@ int dmi_server_wait(dmi_server *S, int64_t id, dmi_request_info *info) {
     std::unique_lock<std::mutex> lk(S->mu);
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) { S->err = "unknown request id"; return DMI_EINVAL; }
-    Request *r = it->second.get();
+    std::shared_ptr<Request> r = it->second;  // another thread's wait may forget the entry first
     S->cv_done.wait(lk, [&] { return r->st == St::Done; });
     if (info) {
         info->status = r->status;
@@ -577,14 +582,25 @@ int dmi_server_wait(dmi_server *S, int64_t id, dmi_request_info *info) {
         info->queue_ms = r->t_admit.time_since_epoch().count() ? ms_between(r->t_submit, r->t_admit) : 0.0;
         info->latency_ms = ms_between(r->t_submit, r->t_done);
     }
-    return r->status;
+    // the request is returned: forget it (its error text stays available for a while)
+    const int status = r->status;
+    if (status != DMI_OK) {
+        S->waited_errors.emplace_back(id, r->err);
+        if (S->waited_errors.size() > kErrorWindow) S->waited_errors.pop_front();
+    }
+    S->reqs.erase(id);
+    return status;
 }
 
 int dmi_server_poll(dmi_server *S, int64_t id, int32_t *done) {
     if (!S || !done) return DMI_EINVAL;
     std::lock_guard<std::mutex> lk(S->mu);
     auto it = S->reqs.find(id);
-    if (it == S->reqs.end()) { S->err = "unknown request id"; return DMI_EINVAL; }
+    if (it == S->reqs.end()) {
+        if (id > 0 && id < S->next_id) { *done = 1; return DMI_OK; }  // already returned by dmi_server_wait
+        S->err = "unknown request id";
+        return DMI_EINVAL;
+    }
     *done = it->second->st == St::Done;
     return DMI_OK;
 }
@@ -619,7 +635,10 @@ const char *dmi_server_request_error(dmi_server *S, int64_t id) {
     if (!S) return "server is NULL";
     std::lock_guard<std::mutex> lk(S->mu);
     auto it = S->reqs.find(id);
-    return it == S->reqs.end() ? "unknown request id" : it->second->err.c_str();
+    if (it != S->reqs.end()) return it->second->err.c_str();
+    for (auto w = S->waited_errors.rbegin(); w != S->waited_errors.rend(); ++w)
+        if (w->first == id) return w->second.c_str();
+    return "unknown request id";
 }
 
 }  // extern "C"

@@ -68,7 +68,8 @@ class T2partOptsC(C.Structure):
 
 class StatsC(C.Structure):
     _fields_ = [("voxels", C.c_int64), ("nnls_solves", C.c_int64), ("fp64_flops", C.c_int64),
-                ("fp32_flops", C.c_int64), ("kernel_ms", C.c_double), ("total_ms", C.c_double)]
+                ("fp32_flops", C.c_int64), ("kernel_ms", C.c_double), ("total_ms", C.c_double),
+                ("voxels_failed", C.c_int64)]
 
 
 class ServerConfigC(C.Structure):

@@ -129,7 +129,10 @@ def T2mapSEcorr(image: np.ndarray, opts: Optional[T2mapOptions] = None, *, alpha
         oo = _lib.T2mapOutC(_ptr(dist), *(_ptr(out.get(k)) for k in keys))
         rc = _lib.load().dmi_t2map_f64(ctx.handle, _ptr(image), C.byref(oc), _ptr(alpha), C.byref(oo))
     _lib.check(rc, ctx.handle)
-    if ctx.stats()["voxels"] == 0 and not opts.Silent:
+    st = ctx.stats()
+    if st["voxels_failed"]:
+        warnings.warn(f"{st['voxels_failed']} voxels were left NaN: a per-voxel search cache overflowed.")
+    if st["voxels"] == 0 and not opts.Silent:
         warnings.warn(f"No voxels found with first-echo signal intensity above threshold {opts.Threshold}.\n"
                       "Consider lowering the threshold or checking the input data.")
     maps.update(out)

@@ -44,6 +44,9 @@ class Server:
         if rc != _lib.DMI_OK:
             raise _lib.DmiError(rc, "dmi_server_create failed (no gfx950 device?)")
         self.handle = h
+        # every request not yet returned by wait(): the library reads and writes its host arrays until then, so
+        # they are kept alive here whether or not the caller keeps the Request
+        self._pending: dict[int, Request] = {}
 
     def _error(self) -> str:
         return _lib.load().dmi_server_last_error(self.handle).decode(errors="replace")
@@ -83,13 +86,16 @@ class Server:
             raise AssertionError(self._error())
         if rc != _lib.DMI_OK:
             raise _lib.DmiError(rc, self._error())
-        return Request(rid.value, opts, image, out, dist, parts, keep=[oc, oo, pc, alpha])
+        req = Request(rid.value, opts, image, out, dist, parts, keep=[oc, oo, pc, alpha])
+        self._pending[req.id] = req
+        return req
 
     def wait(self, req: Request):
         """Blocks until the request finished -> (maps, dist, parts or None, info dict). Raises on failure."""
         info = _lib.RequestInfoC()
         rc = _lib.load().dmi_server_wait(self.handle, req.id, C.byref(info))
         req.done = True
+        self._pending.pop(req.id, None)
         inf = {k: getattr(info, k) for k, _ in _lib.RequestInfoC._fields_ if not k.startswith("_")}
         if rc != _lib.DMI_OK:
             msg = _lib.load().dmi_server_request_error(self.handle, req.id).decode(errors="replace")
@@ -108,8 +114,9 @@ class Server:
 
     def close(self):
         if self.handle:
-            _lib.load().dmi_server_destroy(self.handle)
+            _lib.load().dmi_server_destroy(self.handle)  # waits for every submitted request
             self.handle = None
+        self._pending = {}
 
     def __enter__(self):
         return self

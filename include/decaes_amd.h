@@ -25,7 +25,7 @@ enum {
     DMI_ENOMEM = -3,
     DMI_EHIP = -4,         /* HIP runtime error */
     DMI_ENODEV = -5,       /* no usable gfx950 device */
-    DMI_ECAPACITY = -6     /* a per-voxel device buffer overflowed (reported, never silently truncated) */
+    DMI_ECAPACITY = -6     /* a request does not fit a fixed device capacity (server HBM budget, static scratch) */
 };
 
 /* T2mapOptions.Reg (src/types.jl:66-68) */
@@ -88,6 +88,8 @@ typedef struct {
     int64_t fp32_flops;    /* algorithmic FP32 flops (mixed precision EPG) */
     double kernel_ms;      /* device time of the voxel kernel, HIP events */
     double total_ms;       /* device time of the whole call */
+    int64_t voxels_failed; /* voxels left unwritten (NaN) because a per-voxel search cache overflowed; the L-curve
+                              caches hold every point the search can visit, so this stays 0 */
 } dmi_stats;
 
 /* Context: one HIP stream on `device`, reusable device buffers and cached grid bases.
@@ -182,12 +184,14 @@ int dmi_server_destroy(dmi_server *s);
 int dmi_server_submit(dmi_server *s, const double *image, const dmi_t2map_opts *opts, const double *alpha_in,
                       const dmi_t2map_out *out, const dmi_t2part_opts *popts, double *sfr, double *sgm, double *mfr,
                       double *mgm, int64_t *id);
-/* Blocks until request id has finished (successfully or not). */
+/* Blocks until request id has finished (successfully or not), then forgets it: a request is waited for once.
+ * p50/p99 in dmi_server_get_stats cover the most recent 65536 completed requests. */
 int dmi_server_wait(dmi_server *s, int64_t id, dmi_request_info *info);
 /* *done = 1 once request id has finished. */
 int dmi_server_poll(dmi_server *s, int64_t id, int32_t *done);
 int dmi_server_get_stats(dmi_server *s, dmi_server_stats *out);
 const char *dmi_server_last_error(const dmi_server *s);
+/* Error text of request id (before its wait, or among the 256 most recent failures already waited for). */
 const char *dmi_server_request_error(dmi_server *s, int64_t id);
 
 /* Seeded synthetic multi-echo spin-echo volume (host memory, no device work): per voxel a sum of

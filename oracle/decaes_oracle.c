@@ -433,22 +433,55 @@ static void nnls_ws_free(nnls_ws *ws) {
     free(ws->idx); free(ws->invidx); free(ws->diag);
 }
 
+/* Summation order of the NNLS dot products (a parity-test knob, default 0): 0 = rows in order as the reference;
+ * 1 = rows in reverse order; 2 = two interleaved partial sums (even and odd rows) added at the end. Orders 1 and 2
+ * change only the rounding, as a device reduction tree does; tests use them to find voxels whose result depends on
+ * rounding-level differences in these sums. */
+static int g_sum_order = 0;
+void orc_set_sum_order(int order) { g_sum_order = order; }
+/* Regularization parameter from its logarithm, optionally moved by one ulp (a parity-test knob, default 0): the
+ * search grid of log mu is fixed by the algorithm, so mu = exp(log mu) differs between math libraries only by the
+ * rounding of exp at those points; tests move it by +-1 ulp to find voxels whose result depends on that rounding. */
+static int g_mu_ulp = 0;
+void orc_set_mu_ulp(int dir) { g_mu_ulp = dir; }
+static double mu_exp(double logmu) {
+    double mu = exp(logmu);
+    if (g_mu_ulp > 0) mu = nextafter(mu, INFINITY);
+    else if (g_mu_ulp < 0) mu = nextafter(mu, 0.0);
+    return mu;
+}
+/* sum over i in [lo, hi] of f(i) * g(i) (fma accumulation) in the current order */
+#define ORDERED_DOT(acc, lo, hi, F, G)                                                                      \
+    do {                                                                                                    \
+        if (g_sum_order == 1) {                                                                             \
+            for (int i_ = (hi); i_ >= (lo); --i_) { const int i = i_; acc = fma((F), (G), acc); }           \
+        } else if (g_sum_order == 2) {                                                                      \
+            double a0_ = 0.0, a1_ = 0.0;                                                                    \
+            for (int i_ = (lo); i_ <= (hi); ++i_) {                                                         \
+                const int i = i_;                                                                           \
+                if ((i_ - (lo)) & 1) a1_ = fma((F), (G), a1_); else a0_ = fma((F), (G), a0_);               \
+            }                                                                                               \
+            acc += a0_ + a1_;                                                                               \
+        } else {                                                                                            \
+            SIMD_RED(acc)                                                                                   \
+            for (int i_ = (lo); i_ <= (hi); ++i_) { const int i = i_; acc = fma((F), (G), acc); }           \
+        }                                                                                                   \
+    } while (0)
+
 /* construct_apply_householder! (:259-336). Returns tau (< 0 rejects the column). */
 static double construct_apply_householder(double *A, int lda, double *b1b, int ip, int jp, int m) {
     double *b = b1b - 1;
     if (ip > m) return 0.0;
     double alpha = AX(ip, jp);
     double xnorm = 0.0;
-    SIMD_RED(xnorm)
-    for (int i = ip; i <= m; ++i) xnorm = fma(AX(i, jp), AX(i, jp), xnorm);
+    ORDERED_DOT(xnorm, ip, m, AX(i, jp), AX(i, jp));
     xnorm = sqrt(xnorm);
     if (xnorm == 0) return -1.0;
     double beta = copysign(xnorm, alpha);
     alpha = alpha + beta;
     double tau = alpha / beta;
     double sm = b[ip];
-    SIMD_RED(sm)
-    for (int i = ip + 1; i <= m; ++i) sm = fma(b[i], AX(i, jp) / alpha, sm);
+    ORDERED_DOT(sm, ip + 1, m, b[i], AX(i, jp) / alpha);
     sm *= -tau;
     double A1 = -beta;
     double b1 = b[ip] + sm;
@@ -494,8 +527,7 @@ static void apply_householder_dual(double *A, int lda, double *w1, const double 
     AX(j1, j1) = 1;
     for (int j = j1 + 1; j <= n; ++j) {
         double sm = 0.0;
-        SIMD_RED(sm)
-        for (int i = j1; i <= m1; ++i) sm = fma(AX(i, j), AX(i, j1), sm);
+        ORDERED_DOT(sm, j1, m1, AX(i, j), AX(i, j1));
         sm *= -tau;
         double wj = 0.0;
         AX(j1, j) = AX(j1, j) + sm;
@@ -516,8 +548,7 @@ static void compute_dual(double *w1, const double *A, int lda, const double *b1b
     const double *b = b1b - 1;
     for (int j = j1; j <= n; ++j) {
         double sm = 0.0;
-        SIMD_RED(sm)
-        for (int i = j1; i <= m1; ++i) sm = fma(AX(i, j), b[i], sm);
+        ORDERED_DOT(sm, j1, m1, AX(i, j), b[i]);
         w[j] = sm;
     }
 }
@@ -670,8 +701,8 @@ static void unsafe_nnls(nnls_ws *ws, int tikh, double lam, int init_dual, int ma
     double sm = 0.0;
     int Mlim = tikh ? M : m;
     if (nsetp < Mlim) {
-        SIMD_RED(sm)
-        for (int i = nsetp + 1; i <= Mlim; ++i) { double bi = b[i]; zz[i] = bi; sm = fma(bi, bi, sm); }
+        for (int i = nsetp + 1; i <= Mlim; ++i) zz[i] = b[i];
+        ORDERED_DOT(sm, nsetp + 1, Mlim, b[i], b[i]);
     } else {
         for (int j = 1; j <= n; ++j) w[j] = 0.0;
     }
@@ -820,7 +851,10 @@ static double cur_seminorm_sq(const reg_ws *R) {
     const double *x = R->cx + (size_t)R->ccur * R->n;
     const int *idx = R->cidx + (size_t)R->ccur * R->n;
     double s = 0.0;
-    for (int i = 0; i < R->cnsetp[R->ccur]; ++i) { double v = x[idx[i] - 1]; s += v * v; }
+    if (g_sum_order == 0)
+        for (int i = 0; i < R->cnsetp[R->ccur]; ++i) { double v = x[idx[i] - 1]; s += v * v; }
+    else
+        ORDERED_DOT(s, 0, R->cnsetp[R->ccur] - 1, x[idx[i] - 1], x[idx[i] - 1]);
     return s;
 }
 static double cur_resnorm_sq(const reg_ws *R) {
@@ -978,7 +1012,7 @@ void orc_brent_minimize(orc_fun1 f, void *ctx, double x1, double x2, double xrto
 typedef struct { reg_ws *R; double target; } chi2_ctx;
 static double chi2_relerr_f(double logmu, void *p) {
     chi2_ctx *c = p;
-    cache_solve(c->R, exp(logmu));
+    cache_solve(c->R, mu_exp(logmu));
     double res2 = cur_resnorm_sq(c->R);
     return (res2 - c->target) / c->target;
 }
@@ -1005,7 +1039,7 @@ static void lsq_chi2(reg_ws *R, double chi2_target, double *x_out, double *mu_ou
     else { xf = b; ff = fb; }
     double mu_final, res2_final;
     if (isfinite(ff)) {
-        mu_final = exp(xf);
+        mu_final = mu_exp(xf);
         res2_final = target * (1 + ff);
         memcpy(x_out, cache_solve(R, mu_final), sizeof(double) * n);
     } else {
@@ -1019,7 +1053,7 @@ static void lsq_chi2(reg_ws *R, double chi2_target, double *x_out, double *mu_ou
 typedef struct { reg_ws *R; double delta2; } mdp_ctx;
 static double mdp_f(double logmu, void *p) {
     mdp_ctx *c = p;
-    cache_solve(c->R, exp(logmu));
+    cache_solve(c->R, mu_exp(logmu));
     return cur_resnorm_sq(c->R) - c->delta2;
 }
 static void lsq_mdp(reg_ws *R, double delta, double *x_out, double *mu_out, double *chi2_out) {
@@ -1052,7 +1086,7 @@ static void lsq_mdp(reg_ws *R, double delta, double *x_out, double *mu_out, doub
     else { xf = b; ff = fb; }
     double mu_final, res2_final;
     if (isfinite(ff)) {
-        mu_final = exp(xf);
+        mu_final = mu_exp(xf);
         res2_final = delta * delta + ff;
         memcpy(x_out, cache_solve(R, mu_final), sizeof(double) * n);
     } else {
@@ -1269,7 +1303,7 @@ double orc_gcv_dof(int m, int n, const double *gamma, int ngamma, double lambda)
 typedef struct { reg_ws *R; double low; } gcv_ctx;
 static double loggcv_f(double logmu, void *p) {
     gcv_ctx *c = p;
-    double mu = exp(logmu);
+    double mu = mu_exp(logmu);
     cache_solve(c->R, mu);
     double res2 = cur_resnorm_sq(c->R);
     double dof = orc_gcv_dof(c->R->m, c->R->n, c->R->gamma, c->R->ngamma, mu);
@@ -1285,7 +1319,7 @@ static void lsq_gcv(reg_ws *R, double *x_out, double *mu_out, double *chi2_out) 
     cache_reset(R);
     double xf, yf;
     brent_minimize(loggcv_f, &c, -8.0, 2.0, 0.0, 1e-4, 20, &xf, &yf);
-    double mu_final = exp(xf);
+    double mu_final = mu_exp(xf);
     memcpy(x_out, cache_solve(R, mu_final), sizeof(double) * n);
     double res2 = cur_resnorm_sq(R);
     plain_solve(R);
@@ -1372,6 +1406,12 @@ void orc_lcurve_cache_stats(int out[3]) {
     out[1] = atomic_exchange(&g_lc_max_pc, 0);
     out[2] = atomic_exchange(&g_lc_max_sc, 0);
 }
+/* margin of the last lcurve_corner call on this thread (see the end of orc_lcurve_corner_bt) and, when set, the
+ * per-voxel array the pipeline writes it into (a parity-test output; +Inf for the other regularizations) */
+static _Thread_local double tl_lc_margin = INFINITY;
+static double *g_margin_out = NULL;
+void orc_set_margin_output(double *margin) { g_margin_out = margin; }
+
 /* lcurve_corner (:872-919); backtracking is the reference's keyword (default true) */
 double orc_lcurve_corner_bt(orc_fun2 f, void *ctx, double xlow, double xhigh, double xtol, double Ptol, double Ctol,
                             int backtracking) {
@@ -1424,7 +1464,15 @@ double orc_lcurve_corner_bt(orc_fun2 f, void *ctx, double xlow, double xhigh, do
         if (L.nsc == L.capsc) { L.capsc = L.capsc ? 2 * L.capsc : 64; L.sc = realloc(L.sc, sizeof(lstate) * L.capsc); }
         L.sc[L.nsc++] = st;
     }
-    double xbest = L.pc[pc_argmax(&L)].x;
+    const int ib = pc_argmax(&L);
+    double xbest = L.pc[ib].x;
+    /* relative gap between the best and the second-best curvature: the margin of the final argmax */
+    {
+        double cb = L.pc[ib].C, cs = -INFINITY;
+        for (int i = 0; i < L.npc; ++i)
+            if (i != ib && L.pc[i].C > cs) cs = L.pc[i].C;
+        tl_lc_margin = (isfinite(cb) && isfinite(cs)) ? (cb - cs) / fabs(cb) : INFINITY;
+    }
     atomic_max_int(&g_lc_max_fc, L.nfc);
     atomic_max_int(&g_lc_max_pc, L.npc);
     atomic_max_int(&g_lc_max_sc, L.nsc);
@@ -1438,7 +1486,7 @@ double orc_lcurve_corner(orc_fun2 f, void *ctx, double xlow, double xhigh, doubl
 
 static void lcurve_point(double logmu, void *p, double out[2]) {
     reg_ws *R = p;
-    cache_solve(R, exp(logmu));
+    cache_solve(R, mu_exp(logmu));
     out[0] = log(cur_resnorm_sq(R));
     out[1] = log(cur_seminorm_sq(R));
 }
@@ -1446,7 +1494,7 @@ static void lsq_lcurve(reg_ws *R, double *x_out, double *mu_out, double *chi2_ou
     int n = R->n;
     cache_reset(R);
     double logmu = orc_lcurve_corner(lcurve_point, R, -8.0, 2.0, 1e-4, 1e-4, 1e-4);
-    double mu_final = exp(logmu);
+    double mu_final = mu_exp(logmu);
     memcpy(x_out, cache_solve(R, mu_final), sizeof(double) * n);
     double res2 = cur_resnorm_sq(R);
     plain_solve(R);
@@ -1780,7 +1828,10 @@ static void *pipe_worker(void *arg) {
             double muv, chi2v;
             switch (P->reg) {
             case ORC_REG_NONE: lsq_none(&R, x, &muv, &chi2v); break;
-            case ORC_REG_LCURVE: lsq_lcurve(&R, x, &muv, &chi2v); break;
+            case ORC_REG_LCURVE:
+                lsq_lcurve(&R, x, &muv, &chi2v);
+                if (g_margin_out) g_margin_out[I] = tl_lc_margin;
+                break;
             case ORC_REG_GCV: lsq_gcv(&R, x, &muv, &chi2v); break;
             case ORC_REG_CHI2: lsq_chi2(&R, o->Chi2Factor, x, &muv, &chi2v); break;
             default: {

@@ -110,6 +110,13 @@ void orc_svdvals_bidiag(const double *A, int m, int n, double *s);
 /* GCV singular-value route: 0 = one-sided Jacobi on A (default), 1 = Gram-matrix eigenvalues (diagnostic),
  * 2 = orc_svdvals_bidiag */
 void orc_set_svd_route(int route);
+/* NNLS dot-product summation order: 0 = in order (default), 1 = reversed, 2 = two interleaved partial sums */
+void orc_set_sum_order(int order);
+/* mu = exp(log mu) moved by dir ulps (dir in {-1, 0, 1}; default 0) */
+void orc_set_mu_ulp(int dir);
+/* per-voxel output (volume-sized, indexed by linear voxel index) of the L-curve final-argmax margin: the relative
+ * gap between the best and second-best cached curvature; NULL (default) disables it */
+void orc_set_margin_output(double *margin);
 double orc_gcv_dof(int m, int n, const double *gamma, int ngamma, double lambda);
 
 /* ---- scalar optimizers (src/optimization.jl) with a C callback ---- */

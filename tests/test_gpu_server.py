@@ -81,3 +81,71 @@ def test_server_budget_and_isolation(dmi):
             _same(parts["sfr"], rparts["sfr"])
         st = srv.stats()
     assert st["completed"] == 3 and st["failed"] == 1
+
+
+def test_server_uneven_mix_matches_oracle(dmi, oracle):
+    """A batch drawn like the uneven 64-volume workload, scaled down: matrix sizes from {12x12x2, 16x16x2, 20x20x1},
+    nTE from {24, 32, 48, 56}, TE from {7, 10} ms, mask fraction U(0.2, 0.8), Reg from {chi2, gcv, lcurve}, nT2 40,
+    the default 64 flip angles; all submitted at once through the request loop, every processed voxel of every
+    request checked against the oracle (per-voxel bar on rounding-stable voxels)."""
+    import parity_util as PU
+    from decaes_amd.server import Server
+
+    rng = np.random.default_rng(64)
+    sizes = [(12, 12, 2), (16, 16, 2), (20, 20, 1)]
+    reqs = []
+    for v in range(8):
+        shape = sizes[int(rng.integers(0, 3))]
+        m = int(rng.choice([24, 32, 48, 56]))
+        TE = float(rng.choice([7e-3, 10e-3]))
+        frac = float(rng.uniform(0.2, 0.8))
+        reg = ["chi2", "gcv", "lcurve"][v % 3]
+        img, nin = dmi.synth_mse(shape, m, TE, ncomp=3, mask_fraction=frac, seed=640 + v)
+        opts = dmi.T2mapOptions.from_image(img, TE=TE, nT2=40, T2Range=(10e-3, 2.0), Reg=reg, Chi2Factor=1.02,
+                                           Silent=True, SaveRegParam=True)
+        popts = dmi.T2partOptions.from_t2map(opts, SPWin=(10e-3, 40e-3), MPWin=(40e-3, 2.0))
+        reqs.append((img, opts, popts, nin))
+    print("[server] uneven mix of", len(reqs), "requests", flush=True)
+    with Server(max_streams=4, chunk_voxels=97) as srv:
+        hs = [srv.submit(img, o, part_opts=po) for img, o, po, _ in reqs]
+        results = [srv.wait(h) for h in hs]
+    for v, ((img, o, po, nin), (maps, dist, parts, info)) in enumerate(zip(reqs, results)):
+        assert info["status"] == 0 and info["voxels"] == nin
+        oo = oracle.make_t2map_opts(o.MatrixSize, o.nTE, o.TE, o.nT2, o.T2Range, o.Reg, nRefAngles=o.nRefAngles,
+                                    nRefAnglesMin=o.nRefAnglesMin, Chi2Factor=1.02 if o.Reg == "chi2" else None)
+        omaps, odist, sel, spread = PU.oracle_ensemble(oracle, img, oo, o.T2Range)
+        assert np.array_equal(np.isnan(dist), np.isnan(odist))
+        PU.check(f"server_mix_{v}_{o.Reg}_m{o.nTE}", oracle, o.T2Range, sel, spread, maps, dist, omaps, odist)
+        po_o = oracle.t2part(odist, o.T2Range, (10e-3, 40e-3), (40e-3, 2.0))
+        ok = sel & np.isfinite(po_o["sfr"])
+        assert np.all(np.abs(parts["sfr"][ok] - po_o["sfr"][ok]) <= 1e-3)
+
+
+def test_server_full_size_request(dmi, oracle):
+    """One 240x240x48 volume (1.66M voxels above threshold) through the request loop: whole-volume invariants,
+    the distribution equal to a standalone call, and a seeded sample of voxels against the oracle."""
+    import parity_util as PU
+    from decaes_amd.server import Server
+
+    shape, m, TE, n, T2R = (240, 240, 48), 48, 10e-3, 40, (10e-3, 2.0)
+    print(f"[server] full-size request {shape}", flush=True)
+    img, nin = dmi.synth_mse(shape, m, TE, ncomp=3, mask_fraction=0.6, seed=48)
+    opts = dmi.T2mapOptions.from_image(img, TE=TE, nT2=n, T2Range=T2R, Reg="gcv", nRefAngles=32, Silent=True,
+                                       SaveRegParam=True)
+    popts = dmi.T2partOptions.from_t2map(opts, SPWin=(10e-3, 40e-3), MPWin=(40e-3, 2.0))
+    with Server() as srv:
+        maps, dist, parts, info = srv.wait(srv.submit(img, opts, part_opts=popts))
+    assert info["status"] == 0 and info["voxels"] == nin
+    inmask = img[..., 0] > 0
+    assert np.array_equal(~np.isnan(maps["gdn"]), inmask)
+    d = dist[inmask]
+    g = maps["gdn"][inmask]
+    assert np.all(d >= 0) and np.max(np.abs(d.sum(axis=1) - g) / g) <= 1e-12
+    assert np.all((parts["sfr"][inmask] >= 0) & (parts["sfr"][inmask] <= 1))
+    rmaps, rdist = dmi.T2mapSEcorr(img, opts)
+    _same(dist, rdist)
+    rng = np.random.default_rng(480)
+    vox = np.sort(rng.choice(np.flatnonzero(inmask.ravel(order="F")), size=600, replace=False))
+    oo = oracle.make_t2map_opts(shape, m, TE, n, T2R, "gcv", nRefAngles=32, nRefAnglesMin=opts.nRefAnglesMin)
+    omaps, odist, sel, spread = PU.oracle_ensemble(oracle, img, oo, T2R, vox=vox)
+    PU.check("server_full_size_gcv", oracle, T2R, sel, spread, maps, dist, omaps, odist)

@@ -85,3 +85,18 @@ def test_backtracking_keeps_the_current_state(oracle):
         x_bt = oracle.lcurve_corner(g, -8.0, 2.0, backtracking=True)
         assert len(calls) <= 4 + math.ceil(math.log(10 / 1e-4) / math.log(PHI))
         assert oracle.lcurve_corner(f, -8.0, 2.0, backtracking=False) == x_bt
+
+
+def test_cache_sizes_on_voxels(dmi, oracle):
+    """On real L-curve voxels (the 56-echo, 60-bin, 64-angle shape) the function and point caches never hold more
+    than 4 + 24 = 28 entries and the state cache 24: the fixed 32-entry device caches cannot overflow."""
+    import ctypes as C
+
+    img, _ = dmi.synth_mse((10, 10, 2), 56, 7.9e-3, ncomp=4, mask_fraction=0.8, seed=5)
+    oo = oracle.make_t2map_opts((10, 10, 2), 56, 7.9e-3, 60, (8e-3, 2.0), "lcurve", nRefAngles=64)
+    out = (C.c_int * 3)()
+    oracle.lib().orc_lcurve_cache_stats(out)  # reset
+    oracle.t2map(img, oo, nthreads=8)
+    oracle.lib().orc_lcurve_cache_stats(out)
+    nfc, npc, nsc = out[0], out[1], out[2]
+    assert 4 < nfc <= 28 and 4 < npc <= 28 and nsc <= 24, (nfc, npc, nsc)

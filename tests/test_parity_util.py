@@ -60,3 +60,16 @@ def test_gcv_rounding_ties_are_the_oracles_own(dmi, oracle):
         assert np.all(rel[~tie] < 1e-7), route
         assert np.all(rel[tie] <= 4 * spread["dist"][tie]), route
     del img
+
+
+def test_lcurve_final_choice_ties_are_marked(dmi, oracle):
+    """The L-curve's final argmax margin (relative gap of the best and second-best cached curvature) from the oracle:
+    a voxel of a 24-echo, 40-bin volume whose corner is a 8e-7 tie (the top two curvatures 15.7793421 and 15.7793289)
+    is marked a tie; the margin is finite and positive on every voxel and most voxels are no tie."""
+    shape, m, TE = (20, 20, 1), 24, 7e-3
+    img, _ = dmi.synth_mse(shape, m, TE, ncomp=3, mask_fraction=0.5752288710445501, seed=642)
+    oo = oracle.make_t2map_opts(shape, m, TE, 40, (10e-3, 2.0), "lcurve", nRefAngles=64)
+    omaps, odist, sel, spread = PU.oracle_ensemble(oracle, img, oo, (10e-3, 2.0), nthreads=8)
+    lin = np.ravel_multi_index(np.nonzero(sel), sel.shape, order="F")
+    assert spread["tie"][lin == 257].all()
+    assert 0 < spread["tie"].mean() < 0.6
