@@ -239,7 +239,40 @@ def batch(args):
                     f"latency {info['latency_ms']:.0f} ms")
         wall = time.perf_counter() - t0
         st = srv.stats()
+    alone = None
+    if args.compare_alone:
+        # the same volumes one at a time, device-resident (image and outputs in HBM): the single-volume rate of the
+        # same mix is the total voxel count over the summed kernel times
+        import torch
+
+        from decaes_amd.device import alloc_t2map_outputs, t2map_device, t2part_device
+
+        ctx = dmi.context(0)
+        t_k, t_all, nv = 0.0, 0.0, 0
+        for sp in specs:
+            img, nin = dmi.synth_mse(sp["shape"], sp["nTE"], sp["TE"], ncomp=3, mask_fraction=sp["mask"], seed=sp["seed"])
+            o = dmi.T2mapOptions.from_image(img, TE=sp["TE"], nT2=40, T2Range=(10e-3, 2.0), Reg=sp["Reg"],
+                                            Chi2Factor=1.02, Silent=True)
+            po = dmi.T2partOptions.from_t2map(o, SPWin=(10e-3, 40e-3), MPWin=(40e-3, 2.0))
+            d_img = torch.from_numpy(np.ascontiguousarray(img.transpose(3, 2, 1, 0))).to("cuda:0")
+            out = alloc_t2map_outputs(o, "cuda:0")
+            X, Y, Z = sp["shape"]
+            pout = {k: torch.full((Z, Y, X), float("nan"), dtype=torch.float64, device="cuda:0")
+                    for k in ("sfr", "sgm", "mfr", "mgm")}
+            torch.cuda.synchronize()
+            ta = time.perf_counter()
+            stt = t2map_device(d_img, o, out, ctx=ctx)
+            t2part_device(out["dist"], po, pout, ctx=ctx)
+            torch.cuda.synchronize()
+            t_all += time.perf_counter() - ta
+            t_k += stt["kernel_ms"] * 1e-3
+            nv += stt["voxels"]
+            log(f"[alone] {sp['shape']} nTE={sp['nTE']} {sp['Reg']}: {stt['voxels']} voxels, "
+                f"kernel {stt['kernel_ms']:.0f} ms")
+        alone = {"voxels": nv, "kernel_voxels_per_s": nv / t_k, "device_voxels_per_s": nv / t_all,
+                 "batched_busy_over_alone": (voxels / (st["busy_ms"] * 1e-3)) / (nv / t_all)}
     print(json.dumps({"metric": "voxels/s (T2map+T2part) batched uneven workload", "value": voxels / wall,
+                      "alone": alone,
                       "unit": "voxels/s", "volumes": len(specs), "voxels": voxels, "wall_s": wall,
                       "busy_s": st["busy_ms"] * 1e-3, "voxels_per_s_busy": voxels / (st["busy_ms"] * 1e-3),
                       "host_synthesis_s": t_synth, "voxels_by_reg": per_reg,
@@ -271,7 +304,11 @@ def main():
     ap.add_argument("--batch", action="store_true")
     ap.add_argument("--volumes", type=int, default=64)
     ap.add_argument("--window", type=int, default=6)
+    ap.add_argument("--compare-alone", action="store_true", help="--batch: also run each volume alone on the device")
     args = ap.parse_args()
+    # PyTorch's HIP runtime is loaded before the library, which then binds to it (one runtime per process)
+    import torch  # noqa: F401
+
     if args.copy_probe:
         return copy_probe(args.copy_probe)
     if args.serve:
