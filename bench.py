@@ -407,7 +407,10 @@ def main():
                                f"of the volume (slices {[sl['z'] for sl in slabs]})",
                    "voxels_processed": nvox, "voxels_in_mask": int(nin), "voxels_volume": X * Y * Z,
                    "nnls_solves_per_voxel": float(sum(s["nnls_solves"] for s in stats)) / max(nvox, 1),
-                   "fp64_flops_per_voxel": flops64 / max(nvox, 1), "fp32_flops_per_voxel": flops32 / max(nvox, 1)},
+                   "fp64_flops_per_voxel": flops64 / max(nvox, 1), "fp32_flops_per_voxel": flops32 / max(nvox, 1),
+                   "schedule": {"lds_rows": int(stats[-1]["lds_rows"]), "rows": w["nTE"] + w["nT2"],
+                                "workgroups_per_cu": int(stats[-1]["workgroups_per_cu"]),
+                                "chosen_by": "autotuner (measured on the first warmup slab)"}},
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(args.workload, args.precision),
                      "kernel": f"t2map_kernel<{dmi.api._lib.REG[w['Reg']]}>", "kernel_ms": 1e3 * kernel_s / launches,

@@ -75,7 +75,18 @@ int run_t2map_on(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, con
     HIPCHECK(c, dmi::launch_fill(rb.status, sizeof(int) * 4, 0u, c->stream));
     c->stats = dmi_stats{};
     dmi::T2mapParams p = dmi_host::t2map_params(o, rb.g, d_image, d_alpha_in, d);
+    std::vector<dmi_host::Schedule> cands;
+    int rc0 = dmi_host::t2map_schedules(c->err, o, cands);
+    if (rc0) return rc0;
+    dmi_host::Schedule sched;
+    rc0 = dmi_host::t2map_schedule(c->err, o, sched);
+    if (rc0) return rc0;
+    // the autotuner: with several candidate schedules and none measured yet for these sizes, the first call large
+    // enough runs consecutive slices of its voxel list with each candidate, keeps the fastest for the process and
+    // finishes with it (every schedule gives the same bits, so no work is repeated)
+    bool tune = o->schedule == DMI_SCHED_AUTO && cands.size() > 1 && !dmi_host::schedule_lookup(o, sched);
     bool launched = false;
+    double tune_ms = 0.0;
     for (int64_t v0 = 0; v0 < V; v0 += rb.vox_cap) {
         const int64_t v1 = std::min<int64_t>(V, v0 + rb.vox_cap);
         HIPCHECK(c, dmi::launch_fill(rb.counter, sizeof(unsigned), 0u, c->stream));
@@ -85,10 +96,40 @@ int run_t2map_on(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, con
         HIPCHECK(c, hipStreamSynchronize(c->stream));
         c->stats.voxels += nvox;
         if (nvox == 0) continue;
+        int64_t done = 0;
+        if (tune) {
+            int64_t need = 0;
+            for (const auto &s : cands) need += 8ll * c->num_cu * s.per_cu;
+            if ((int64_t)nvox >= 2 * need) {
+                double best_rate = -1.0;
+                for (const auto &s : cands) {
+                    const int64_t cnt = 8ll * c->num_cu * s.per_cu;
+                    dmi::T2mapParams ps = p;
+                    dmi_host::apply_schedule(ps, s);
+                    HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
+                    int rc = dmi_host::t2map_launch_raw(c->err, ps, rb.vox + done, cnt, rb.counter, rb.counters,
+                                                        rb.status, rb.slots, rb.slot_doubles, c->num_cu, c->stream);
+                    if (rc) return rc;
+                    HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
+                    HIPCHECK(c, hipEventSynchronize(c->ev[2]));
+                    float ms = 0;
+                    HIPCHECK(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
+                    tune_ms += ms;
+                    const double rate = cnt / std::max(ms, 1e-3f);
+                    if (rate > best_rate) { best_rate = rate; sched = s; }
+                    done += cnt;
+                }
+                dmi_host::schedule_store(o, sched);
+                tune = false;
+            }
+        }
         if (!launched) HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
         launched = true;
-        int rc = dmi_host::t2map_launch_raw(c->err, p, rb.vox, nvox, rb.counter, rb.counters, rb.status, rb.slots,
-                                            rb.slot_doubles, c->num_cu, c->stream);
+        dmi_host::apply_schedule(p, sched);
+        c->stats.lds_rows = std::min(p.rl, p.mw);
+        c->stats.workgroups_per_cu = sched.per_cu;
+        int rc = dmi_host::t2map_launch_raw(c->err, p, rb.vox + done, nvox - done, rb.counter, rb.counters, rb.status,
+                                            rb.slots, rb.slot_doubles, c->num_cu, c->stream);
         if (rc) return rc;
     }
     unsigned long long cnt[4] = {0, 0, 0, 0};
@@ -107,7 +148,7 @@ int run_t2map_on(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, con
     c->stats.fp64_flops = (int64_t)cnt[0];
     c->stats.fp32_flops = (int64_t)cnt[2];
     c->stats.voxels_failed = (int64_t)cnt[3];
-    c->stats.kernel_ms = kms;
+    c->stats.kernel_ms = kms + tune_ms;
     c->stats.total_ms = tms;
     (void)status;
     return DMI_OK;
@@ -125,13 +166,14 @@ int run_t2map(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, const 
     HIPCHECK(c, c->lb.counter.ensure(sizeof(unsigned) * 4));
     HIPCHECK(c, c->lb.counters.ensure(sizeof(unsigned long long) * 4));
     HIPCHECK(c, c->lb.status.ensure(sizeof(int) * 4));
-    // slots for the largest persistent grid the voxel kernel can run with these options
-    const int lda = dmi::t2map_lda(o->nTE, o->nT2);
-    const int K = std::isnan(o->SetFlipAngle) ? o->nRefAngles : 1;
-    int per_cu = 0;
-    HIPCHECK(c, dmi::t2map_occupancy(o->Reg, &per_cu, dmi::t2map_lds_bytes(o->nTE, o->nT2, lda, K)));
-    const int64_t slot_doubles = dmi::t2map_slot_doubles(o->nTE, o->nT2) *
-                                 std::max<int64_t>(1, std::min<int64_t>((int64_t)c->num_cu * std::max(per_cu, 1), V));
+    // slots for the largest persistent grid the voxel kernel can run with these options, under any schedule
+    std::vector<dmi_host::Schedule> cands;
+    rc = dmi_host::t2map_schedules(c->err, o, cands);
+    if (rc) return rc;
+    int64_t slot_doubles = 0;
+    for (const auto &s : cands)
+        slot_doubles = std::max<int64_t>(slot_doubles, dmi::t2map_slot_doubles(o->nTE, o->nT2, s.rl >= o->nTE + o->nT2 ? 0 : s.gld) *
+                                                           std::max<int64_t>(1, std::min<int64_t>((int64_t)c->num_cu * std::max(s.per_cu, 1), V)));
     HIPCHECK(c, c->lb.scratch.ensure(sizeof(double) * (size_t)slot_doubles));
     RunBufs rb{c->grids.ptrs(), c->vox.as<int64_t>(), std::max<int64_t>(V, 1), c->lb.counter.as<unsigned>(),
                c->lb.counters.as<unsigned long long>(), c->lb.status.as<int>(), c->lb.scratch.as<double>(), slot_doubles};

@@ -7,6 +7,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cstdarg>
 #include <cstdio>
 
@@ -126,6 +129,7 @@ int validate_t2map(std::string &err, const dmi_t2map_opts *o) {
     if (std::isnan(o->SetFlipAngle) && o->nRefAngles > dmi::kMaxAngles)
         return errorf(err, DMI_EUNSUPPORTED, "nRefAngles = %d exceeds the device limit of %d", o->nRefAngles, dmi::kMaxAngles);
     if (o->precision != DMI_PREC_FP64 && o->precision != DMI_PREC_MIXED) return errorf(err, DMI_EINVAL, "unknown precision %d", o->precision);
+    if (o->schedule < DMI_SCHED_AUTO || o->schedule > DMI_SCHED_SPLIT) return errorf(err, DMI_EINVAL, "unknown schedule %d", o->schedule);
     return DMI_OK;
 }
 
@@ -230,6 +234,8 @@ dmi::T2mapParams t2map_params(const dmi_t2map_opts *o, const GridPtrs &g, const 
     p.fp32 = o->precision == DMI_PREC_MIXED;
     p.lda = dmi::t2map_lda(m, n);
     p.mw = m + n;
+    p.rl = p.lda;  // all rows in LDS unless a schedule says otherwise
+    p.gld = 0;
     p.save_resnorm = o->SaveResidualNorm; p.save_decaycurve = o->SaveDecayCurve;
     p.save_regparam = o->SaveRegParam; p.save_basis = o->SaveNNLSBasis;
     p.V = o->MatrixSize[0] * o->MatrixSize[1] * o->MatrixSize[2];
@@ -263,7 +269,7 @@ int t2map_launch_raw(std::string &err, dmi::T2mapParams p, const int64_t *d_vox,
     int per_cu = 0;
     DMI_HIP(err, dmi::t2map_occupancy(p.reg, &per_cu, lds));
     if (per_cu < 1) return errorf(err, DMI_EUNSUPPORTED, "voxel kernel does not fit on a CU (LDS %zu bytes)", lds);
-    p.slot_stride = dmi::t2map_slot_doubles(p.m, p.n);
+    p.slot_stride = dmi::t2map_slot_doubles(p.m, p.n, p.gld);
     const int64_t fit = slot_doubles / p.slot_stride;
     if (fit < 1) return errorf(err, DMI_ECAPACITY, "scratch of %lld doubles holds no voxel slot of %lld doubles",
                                (long long)slot_doubles, (long long)p.slot_stride);
@@ -290,10 +296,84 @@ int t2map_launch(std::string &err, dmi::T2mapParams p, const int64_t *d_vox, int
     DMI_HIP(err, dmi::t2map_occupancy(p.reg, &per_cu, lds));
     if (per_cu < 1) return errorf(err, DMI_EUNSUPPORTED, "voxel kernel does not fit on a CU (LDS %zu bytes)", lds);
     const int64_t grid = std::min<int64_t>((int64_t)num_cu * per_cu, nvox);
-    const int64_t slot_doubles = dmi::t2map_slot_doubles(p.m, p.n) * grid;
+    const int64_t slot_doubles = dmi::t2map_slot_doubles(p.m, p.n, p.gld) * grid;
     DMI_HIP(err, lb.scratch.ensure(sizeof(double) * (size_t)slot_doubles));
     return t2map_launch_raw(err, p, d_vox, nvox, lb.counter.as<unsigned>(), lb.counters.as<unsigned long long>(),
                             lb.status.as<int>(), lb.scratch.as<double>(), slot_doubles, num_cu, s);
+}
+
+int t2map_schedules(std::string &err, const dmi_t2map_opts *o, std::vector<Schedule> &out) {
+    out.clear();
+    const int m = o->nTE, n = o->nT2, K = std::isnan(o->SetFlipAngle) ? o->nRefAngles : 1;
+    Schedule full;
+    full.lda = dmi::t2map_lda(m, n);
+    full.rl = full.lda;
+    full.gld = 0;
+    DMI_HIP(err, dmi::t2map_occupancy(o->Reg, &full.per_cu, dmi::t2map_lds_bytes(m, n, full.lda, K)));
+    out.push_back(full);
+    // Only the Tikhonov-regularized solves append rows beyond m; GCV also keeps its Gram matrix in rows [m, m + n)
+    if (o->Reg != DMI_REG_LCURVE && o->Reg != DMI_REG_CHI2 && o->Reg != DMI_REG_MDP) return DMI_OK;
+    // the basis rows, and every row a triangular factor or a reflector head can reach (< n + 2), stay in LDS
+    int rmin = std::max(dmi::basis_ld(m), n + 2);
+    rmin += (rmin % 4 == 0) ? 2 : 0;  // lda = 2 mod 4 (bank spread of 16-byte column loads)
+    int best = full.per_cu;
+    for (int target = full.per_cu + 1; target <= full.per_cu + 2; ++target) {
+        for (int lda = full.lda - 4; lda >= rmin; lda -= 4) {
+            int pc = 0;
+            DMI_HIP(err, dmi::t2map_occupancy(o->Reg, &pc, dmi::t2map_lds_bytes(m, n, lda, K)));
+            if (pc >= target) {
+                if (pc > best) {
+                    Schedule s;
+                    s.lda = lda;
+                    s.rl = lda;
+                    s.gld = ((m + n - lda) + 1) & ~1;
+                    s.per_cu = pc;
+                    out.push_back(s);
+                    best = pc;
+                }
+                break;
+            }
+        }
+    }
+    return DMI_OK;
+}
+
+namespace {
+std::mutex g_sched_mu;
+std::map<std::tuple<int, int, int, int, int, int>, Schedule> g_sched;
+std::tuple<int, int, int, int, int, int> sched_key(const dmi_t2map_opts *o) {
+    const int K = std::isnan(o->SetFlipAngle) ? o->nRefAngles : 1;
+    return {o->nTE, o->nT2, K, o->Reg, o->precision, o->RefConAngle != 180.0};
+}
+}  // namespace
+
+bool schedule_lookup(const dmi_t2map_opts *o, Schedule &s) {
+    std::lock_guard<std::mutex> lk(g_sched_mu);
+    auto it = g_sched.find(sched_key(o));
+    if (it == g_sched.end()) return false;
+    s = it->second;
+    return true;
+}
+
+void schedule_store(const dmi_t2map_opts *o, const Schedule &s) {
+    std::lock_guard<std::mutex> lk(g_sched_mu);
+    g_sched[sched_key(o)] = s;
+}
+
+int t2map_schedule(std::string &err, const dmi_t2map_opts *o, Schedule &s) {
+    std::vector<Schedule> c;
+    int rc = t2map_schedules(err, o, c);
+    if (rc) return rc;
+    if (o->schedule == DMI_SCHED_SPLIT) { s = c.back(); return DMI_OK; }  // the most workgroups per CU
+    if (o->schedule == DMI_SCHED_AUTO && schedule_lookup(o, s)) return DMI_OK;
+    s = c.front();
+    return DMI_OK;
+}
+
+void apply_schedule(dmi::T2mapParams &p, const Schedule &s) {
+    p.lda = s.lda;
+    p.rl = s.rl;
+    p.gld = s.rl >= p.mw ? 0 : s.gld;
 }
 
 int t2part_prepare(std::string &err, const dmi_t2part_opts *o, std::vector<double> &lg, std::vector<double> &wts,

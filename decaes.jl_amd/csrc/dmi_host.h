@@ -76,6 +76,21 @@ int validate_t2map(std::string &err, const dmi_t2map_opts *o);
 // Output-pointer checks for the options' Save* flags.
 int validate_outputs(std::string &err, const dmi_t2map_opts *o, const dmi_t2map_out *d);
 
+// Launch schedule of the voxel kernel: the LDS column stride lda of the NNLS workspace, the rows rl held in LDS
+// (rows [rl, m + n) live in the workgroup's global slot with column stride gld) and the workgroups that fit a CU.
+struct Schedule {
+    int lda = 0, rl = 0, gld = 0, per_cu = 0;
+};
+// Candidates for these options: the all-LDS schedule first, then, for the Tikhonov-regularized methods (whose
+// workspace has m + n rows), split schedules with the largest LDS part that fits one and two more workgroups per CU.
+int t2map_schedules(std::string &err, const dmi_t2map_opts *o, std::vector<Schedule> &out);
+// The process-wide choice measured for these sizes and method (false if none yet).
+bool schedule_lookup(const dmi_t2map_opts *o, Schedule &s);
+void schedule_store(const dmi_t2map_opts *o, const Schedule &s);
+// The schedule a launch of these options uses: the caller's explicit choice, else the measured one, else all-LDS.
+int t2map_schedule(std::string &err, const dmi_t2map_opts *o, Schedule &s);
+void apply_schedule(dmi::T2mapParams &p, const Schedule &s);
+
 // Kernel parameters for one request whose image, B1 map and outputs are device arrays; the voxel list and
 // count are set per launch.
 dmi::T2mapParams t2map_params(const dmi_t2map_opts *o, const GridPtrs &g, const double *d_image, const double *d_alpha,

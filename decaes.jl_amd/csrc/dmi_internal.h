@@ -54,7 +54,8 @@ struct GridParams {
 struct T2mapParams {
     int m, n, K, mineval, reg, mode, general;
     int fp32;             // 1: FP32 EPG recursion for the voxel basis (mixed precision)
-    int lda, mw;          // LDS workspace: mw = m + n rows, column-major with column stride lda = t2map_lda(m, n)
+    int lda, mw;          // workspace: mw = m + n rows, column-major; rows [0, rl) in LDS with column stride lda
+    int rl, gld;          // rows [rl, mw) in the workgroup's global slot, column stride gld (rl >= mw: all in LDS)
     int save_resnorm, save_decaycurve, save_regparam, save_basis;
     int64_t V;            // voxels in the volume
     int64_t nvox;         // voxels above threshold
@@ -88,7 +89,7 @@ struct T2partParams {
 
 int t2map_lda(int m, int n);
 size_t t2map_lds_bytes(int m, int n, int lda, int K);  // K: flip-angle grid size of the launch (1 with SetFlipAngle)
-int64_t t2map_slot_doubles(int m, int n);
+int64_t t2map_slot_doubles(int m, int n, int gld);
 
 hipError_t launch_grid_basis(const GridParams &p, hipStream_t s);
 hipError_t launch_transpose(const double *src, double *dst, int m, int n, int K, hipStream_t s);

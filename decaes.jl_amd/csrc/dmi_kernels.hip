@@ -35,6 +35,8 @@ typedef const __attribute__((address_space(1))) int64_t cgi64;
 typedef double dv2 __attribute__((ext_vector_type(2)));
 typedef const __attribute__((address_space(3))) dv2 cldv2;
 typedef __attribute__((address_space(3))) dv2 ldv2;
+typedef const __attribute__((address_space(1))) dv2 cgv2;
+typedef __attribute__((address_space(1))) dv2 gv2;
 
 constexpr double kDeg2Rad = 3.141592653589793 / 180.0;  // Julia deg2rad: x * (Float64(pi)/180)
 constexpr double kPi180Hi = 0.017453292519943295;      // pi/180 as a double-double
@@ -545,6 +547,12 @@ struct Wave {
     ldbl *A, *b, *zz, *x, *xf, *b0, *cmu, *crn, *csemi, *cx;
     lint *idx, *diag, *stk;
     int lda, mw, m, n, L;
+    // Rows [0, rl) of the workspace are in LDS (column stride lda); rows [rl, mw) of column j, when rl < mw, are in
+    // this workgroup's global slot at G[j * gld + (r - rl)] (the split schedule, which lets more workgroups share a
+    // CU's LDS). rl is even and at least max(basis_ld(m), n + 2): the basis rows and every row of the triangular
+    // factor and of a reflector's head stay in LDS; only Tikhonov rows appended late live in G.
+    int rl, gld;
+    gdbl *G;
     // the dual w_j of position j = L (the reference's w), in a register: only this lane writes or reads it
     double wl;
     // per-workgroup global scratch
@@ -556,6 +564,12 @@ struct Wave {
     bool failed;  // this voxel gets no outputs (a per-voxel cache overflowed)
 
     __device__ __forceinline__ ldbl &Aat(int i, int j) { return A[j * lda + i]; }
+    // row i of column j wherever it lives (i < rl: LDS, else the global slot); per-lane rows may differ
+    __device__ __forceinline__ double Aget(int i, int j) { return (i < rl) ? (double)A[j * lda + i] : (double)G[j * gld + (i - rl)]; }
+    __device__ __forceinline__ void Aput(int i, int j, double v) {
+        if (i < rl) A[j * lda + i] = v;
+        else G[j * gld + (i - rl)] = v;
+    }
 
     // -------------------------------- Lawson-Hanson NNLS (src/NNLS.jl) --------------------------------
     // construct_apply_householder! (:259-336), 0-based: pivot row ip, column jp, rows [0, mr). Lane L holds
@@ -564,17 +578,15 @@ struct Wave {
     __device__ __forceinline__ double construct_apply_householder(int ip, int jp, int mr) {
         if (ip >= mr) return 0.0;
         const ldbl *cp = A + jp * lda;
-        ldbl *ci = A + ip * lda;
-        ldbl *cj = A + jp * lda;
         // t: column ip, which moves to column jp when the pivot column is swapped into place (read up front;
         // nothing writes it before the swap)
         double c[3], bv[3], t[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const int i = L + k * kWave;
-            c[k] = (i < mr) ? cp[i] : 0.0;
+            c[k] = (i < mr) ? Aget(i, jp) : 0.0;
             bv[k] = (i < mr) ? b[i] : 0.0;
-            t[k] = (ip != jp && i < mr) ? ci[i] : 0.0;
+            t[k] = (ip != jp && i < mr) ? Aget(i, ip) : 0.0;
         }
         double alpha = rfl(cp[ip]);
         double part = 0.0;
@@ -615,8 +627,8 @@ struct Wave {
             for (int k = 0; k < 3; ++k) {
                 const int i = L + k * kWave;
                 if (i >= mr) continue;
-                ci[i] = (i < ip) ? c[k] : ((i == ip) ? A1 : u[k]);
-                if (ip != jp) cj[i] = t[k];
+                Aput(i, ip, (i < ip) ? c[k] : ((i == ip) ? A1 : u[k]));
+                if (ip != jp) Aput(i, jp, t[k]);
                 b[i] = (i < ip) ? bv[k] : ((i == ip) ? b1 : fma(sm, u[k], bv[k]));
             }
             flops += 3ull * (mr - ip);
@@ -626,7 +638,7 @@ struct Wave {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const int i = L + k * kWave;
-                    if (i < mr) { ci[i] = c[k]; cj[i] = t[k]; }
+                    if (i < mr) { Aput(i, ip, c[k]); Aput(i, jp, t[k]); }
                 }
         }
         lds_order();
@@ -647,10 +659,12 @@ struct Wave {
             ldv2 *vj = (ldv2 *)cj;
             const int lo = j1 + 1, hi = m1;
             const int h = lo & 1, bs = lo + h, t = (hi - bs) & 1, be = hi - t;
-            const int q0 = bs >> 1, q1 = be >> 1;
+            const int q0 = bs >> 1, qe = be >> 1;
+            // pairs [q0, q1) are in LDS, pairs [qg, qe) in the global slot (split schedule; empty otherwise)
+            const int qR = rl >> 1, q1 = min(qe, qR), qg = max(q0, qR);
             double s0 = cj[j1], s1 = 0.0, s2 = 0.0, s3 = 0.0;
             if (h) s1 = cj[lo] * cu[lo];
-            if (t) s2 = cj[hi - 1] * cu[hi - 1];
+            if (t) s2 = Aget(hi - 1, j) * Aget(hi - 1, j1);
             int q = q0;
             // eight row pairs per step (all sixteen loads issued before the first use), then four, two, one;
             // accumulator k always takes the same row of a step, as a plain two-pair loop would
@@ -680,6 +694,26 @@ struct Wave {
                 const dv2 a0 = vj[q], u0 = vu[q];
                 s0 = fma(a0.x, u0.x, s0);
                 s1 = fma(a0.y, u0.y, s1);
+            }
+            // global pairs continue the same accumulator pattern: pair q goes to (s0, s1) when q - q0 is even
+            if (qg < qe) {
+                cgv2 *gj = (cgv2 *)(G + (size_t)j * gld), *gu = (cgv2 *)(G + (size_t)j1 * gld);
+                int r = qg;
+                auto acc = [&](const dv2 &a, const dv2 &u, int pq) {
+                    if (((pq - q0) & 1) == 0) { s0 = fma(a.x, u.x, s0); s1 = fma(a.y, u.y, s1); }
+                    else { s2 = fma(a.x, u.x, s2); s3 = fma(a.y, u.y, s3); }
+                };
+                for (; r + 8 <= qe; r += 8) {
+                    dv2 a[8], u[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) { a[k] = gj[r + k - qR]; u[k] = gu[r + k - qR]; }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) acc(a[k], u[k], r + k);
+                }
+                for (; r < qe; ++r) {
+                    const dv2 a = gj[r - qR], u = gu[r - qR];
+                    acc(a, u, r);
+                }
             }
             const double sm = ((s0 + s1) + (s2 + s3)) * -tau;
             cj[j1] = cj[j1] + sm;
@@ -718,10 +752,38 @@ struct Wave {
                 w1 = fma(a.y, bb.y, w1);
                 vj[q] = a;
             }
+            if (qg < qe) {
+                gv2 *gj = (gv2 *)(G + (size_t)j * gld);
+                cgv2 *gu = (cgv2 *)(G + (size_t)j1 * gld);
+                int r = qg;
+                for (; r + 8 <= qe; r += 8) {
+                    dv2 a[8], u[8], bb[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) { a[k] = gj[r + k - qR]; u[k] = gu[r + k - qR]; bb[k] = vb[r + k]; }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        a[k].x = fma(sm, u[k].x, a[k].x);
+                        a[k].y = fma(sm, u[k].y, a[k].y);
+                        w0 = fma(a[k].x, bb[k].x, w0);
+                        w1 = fma(a[k].y, bb[k].y, w1);
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) gj[r + k - qR] = a[k];
+                }
+                for (; r < qe; ++r) {
+                    dv2 a = gj[r - qR];
+                    const dv2 u = gu[r - qR], bb = vb[r];
+                    a.x = fma(sm, u.x, a.x);
+                    a.y = fma(sm, u.y, a.y);
+                    w0 = fma(a.x, bb.x, w0);
+                    w1 = fma(a.y, bb.y, w1);
+                    gj[r - qR] = a;
+                }
+            }
             if (t) {
-                const double a = fma(sm, cu[hi - 1], cj[hi - 1]);
+                const double a = fma(sm, Aget(hi - 1, j1), Aget(hi - 1, j));
                 w1 = fma(a, b[hi - 1], w1);
-                cj[hi - 1] = a;
+                Aput(hi - 1, j, a);
             }
             wl = w0 + w1;
         }
@@ -739,9 +801,9 @@ struct Wave {
             const int h = lo & 1, bs = lo + h, t = (hi - bs) & 1, be = hi - t;
             double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
             if (h) s0 = cj[lo] * b[lo];
-            if (t) s1 = cj[hi - 1] * b[hi - 1];
+            if (t) s1 = Aget(hi - 1, j) * b[hi - 1];
             int q = bs >> 1;
-            const int q1 = be >> 1;
+            const int q0 = q, qe = be >> 1, qR = rl >> 1, q1 = min(qe, qR), qg = max(q0, qR);
             // four row pairs per step (all eight loads issued before the first use), then two, then one;
             // accumulator k always takes the same row of a step, as a plain two-pair loop would
             for (; q + 4 <= q1; q += 4) {
@@ -759,6 +821,26 @@ struct Wave {
                 const dv2 a0 = vj[q], u0 = vb[q];
                 s0 = fma(a0.x, u0.x, s0);
                 s1 = fma(a0.y, u0.y, s1);
+            }
+            // global pairs (split schedule) continue the pattern: pair q goes to (s0, s1) when q - q0 is even
+            if (qg < qe) {
+                cgv2 *gj = (cgv2 *)(G + (size_t)j * gld);
+                auto acc = [&](const dv2 &a, const dv2 &u, int pq) {
+                    if (((pq - q0) & 1) == 0) { s0 = fma(a.x, u.x, s0); s1 = fma(a.y, u.y, s1); }
+                    else { s2 = fma(a.x, u.x, s2); s3 = fma(a.y, u.y, s3); }
+                };
+                int r = qg;
+                for (; r + 8 <= qe; r += 8) {
+                    dv2 a[8], u[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) { a[k] = gj[r + k - qR]; u[k] = vb[r + k]; }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) acc(a[k], u[k], r + k);
+                }
+                for (; r < qe; ++r) {
+                    const dv2 a = gj[r - qR], u = vb[r];
+                    acc(a, u, r);
+                }
             }
             wl = (s0 + s1) + (s2 + s3);
         } else if (j >= j1 && j < n) {
@@ -823,7 +905,7 @@ struct Wave {
     __device__ __forceinline__ void nnls_delete(int &imv, int &nsetp, int mrows, int &pidx, double &xp) {
         for (;;) {
             if (L == imv) xp = 0.0;
-            for (int r = imv + 1 + L; r < mrows; r += kWave) Aat(r, imv) = 0.0;
+            for (int r = imv + 1 + L; r < mrows; r += kWave) Aput(r, imv, 0.0);
             lds_order();
             if (imv != nsetp - 1) {
                 for (int i = imv + 1; i <= nsetp - 1; ++i) {
@@ -848,9 +930,16 @@ struct Wave {
                     lds_order();
                 }
                 for (int r = L; r < mrows; r += kWave) {
-                    double t = Aat(r, imv);
-                    for (int j = imv; j < nsetp - 1; ++j) Aat(r, j) = Aat(r, j + 1);
-                    Aat(r, nsetp - 1) = t;
+                    if (r < rl) {
+                        double t = Aat(r, imv);
+                        for (int j = imv; j < nsetp - 1; ++j) Aat(r, j) = Aat(r, j + 1);
+                        Aat(r, nsetp - 1) = t;
+                    } else {
+                        gdbl *gr = G + (r - rl);
+                        double t = gr[(size_t)imv * gld];
+                        for (int j = imv; j < nsetp - 1; ++j) gr[(size_t)j * gld] = gr[(size_t)(j + 1) * gld];
+                        gr[(size_t)(nsetp - 1) * gld] = t;
+                    }
                 }
                 // positions imv + 1 .. nsetp - 1 move down by one; position imv goes last
                 const int pim = readlane(pidx, imv);
@@ -899,7 +988,7 @@ struct Wave {
                 cmax = readlane(pidx, jmax);
                 const bool newdiag = tikh && !((dmask >> cmax) & 1ull);
                 if (newdiag) {
-                    if (L == 0) Aat(mr, jmax) = lam;
+                    if (L == 0) Aput(mr, jmax, lam);
                     lds_order();
                 }
                 const int mrh = tikh ? min(mr + 1, M) : mr;
@@ -910,7 +999,7 @@ struct Wave {
                 }
                 if (tau >= 0) break;
                 if (L == jmax) { wv = 0.0; wl = 0.0; }
-                if (L == 0 && tikh && mr < M) Aat(mr, jmax) = 0.0;
+                if (L == 0 && tikh && mr < M) Aput(mr, jmax, 0.0);
                 lds_order();
             }
             DMI_LAP(kPhSelect);
@@ -1035,8 +1124,10 @@ struct Wave {
         if (L < m) zz[L] = b0[L] - a0 * xj;
         if (L + kWave < m) zz[L + kWave] = b0[L + kWave] - a1 * xj;
         // Tikhonov rows below the copied ones are cleared while the copy is in flight; a pad row (odd m) after it
-        if (tikh && L < n)
-            for (int i = ldg; i < M; ++i) Aat(i, L) = 0.0;
+        if (tikh && L < n) {
+            for (int i = ldg; i < min(M, rl); ++i) Aat(i, L) = 0.0;
+            for (int i = max(ldg, rl); i < M; ++i) G[(size_t)L * gld + (i - rl)] = 0.0;
+        }
         basis_wait();
         if (tikh && L < n && ldg > m) Aat(m, L) = 0.0;
         __syncthreads();
@@ -2085,11 +2176,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     W.n = p.n;
     W.lda = p.lda;
     W.mw = p.mw;
+    W.rl = p.rl;
+    W.gld = p.gld;
     ldbl *s = (ldbl *)smem;
     // every double array has an even length, so each starts 16-byte aligned
+    const int bz = (p.mw + 1) & ~1;
     W.A = s; s += (size_t)p.n * p.lda;
-    W.b = s; s += p.lda;
-    W.zz = s; s += p.lda;
+    W.b = s; s += bz;
+    W.zz = s; s += bz;
     const int ne = (p.n + 1) & ~1;
     W.x = s; s += ne;
     W.xf = s; s += ne;
@@ -2107,6 +2201,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     // L-curve cache
     W.A0s = slot;
     W.lc = slot + (int64_t)basis_ld(p.m) * p.n;
+    W.G = W.lc + 7 * kLcurveCap;  // workspace rows [rl, mw) of the split schedule, n columns of stride gld
     W.flops = 0;
     W.flops32 = 0;
     W.solves = 0;
@@ -2284,15 +2379,16 @@ int t2map_lda(int m, int n) {
 }
 
 size_t t2map_lds_bytes(int m, int n, int lda, int K) {
-    const size_t ne = (size_t)((n + 1) & ~1);
-    size_t d = (size_t)n * lda + 2 * (size_t)lda + 2 * ne + (size_t)((m + 1) & ~1) + 3 * kTikhCache +
+    const size_t ne = (size_t)((n + 1) & ~1), bz = (size_t)((m + n + 1) & ~1);
+    size_t d = (size_t)n * lda + 2 * bz + 2 * ne + (size_t)((m + 1) & ~1) + 3 * kTikhCache +
                (size_t)kTikhCache * n;
     size_t i = 2 * ne + 3 * ((size_t)K + 2);
     return d * sizeof(double) + i * sizeof(int);
 }
 
-int64_t t2map_slot_doubles(int m, int n) {
-    int64_t s = (int64_t)basis_ld(m) * n + 7 * (int64_t)kLcurveCap;  // basis, function cache (3), point cache (4)
+int64_t t2map_slot_doubles(int m, int n, int gld) {
+    // basis, function cache (3), point cache (4), the workspace rows kept outside LDS (split schedule)
+    int64_t s = (int64_t)basis_ld(m) * n + 7 * (int64_t)kLcurveCap + (int64_t)n * gld;
     return (s + 31) / 32 * 32;
 }
 

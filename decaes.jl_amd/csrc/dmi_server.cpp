@@ -257,7 +257,14 @@ int bind_and_select(dmi_server *S, StreamState &st, Request *r) {
         DMI_HIP(e, dmi::t2map_occupancy(reg, &per_cu, lds));
         grid = std::max(grid, per_cu * S->num_cu);
     }
-    DMI_HIP(e, st.lb.scratch.ensure(sizeof(double) * (size_t)dmi::t2map_slot_doubles(m, n) * std::max(grid, 1)));
+    // the request's launch schedule (its explicit choice, or the one measured for these sizes, or all-LDS)
+    dmi_host::Schedule sched;
+    rc = dmi_host::t2map_schedule(e, &r->o, sched);
+    if (rc) return rc;
+    const int gld = sched.rl >= m + n ? 0 : sched.gld;
+    const size_t slots = std::max<size_t>((size_t)dmi::t2map_slot_doubles(m, n, 0) * std::max(grid, 1),
+                                          (size_t)dmi::t2map_slot_doubles(m, n, gld) * std::max(sched.per_cu * S->num_cu, 1));
+    DMI_HIP(e, st.lb.scratch.ensure(sizeof(double) * slots));
     DMI_HIP(e, st.lb.counter.ensure(sizeof(unsigned) * 4));
     DMI_HIP(e, st.lb.counters.ensure(sizeof(unsigned long long) * 4));
     DMI_HIP(e, st.lb.status.ensure(sizeof(int) * 4));
@@ -268,6 +275,7 @@ int bind_and_select(dmi_server *S, StreamState &st, Request *r) {
     dmi_t2map_out d{r->d_out[0], r->d_out[1], r->d_out[2], r->d_out[3], r->d_out[4], r->d_out[5],
                     r->d_out[6], r->d_out[7], r->d_out[8], r->d_out[9], r->d_out[10], r->d_out[11]};
     r->params = dmi_host::t2map_params(&r->o, st.grids.ptrs(), r->d_image, r->d_alpha, &d);
+    dmi_host::apply_schedule(r->params, sched);
     return DMI_OK;
 }
 

@@ -22,6 +22,7 @@ DMI_ECAPACITY = -6
 
 REG = {"none": 0, "lcurve": 1, "gcv": 2, "chi2": 3, "mdp": 4}
 PREC = {"fp64": 0, "mixed": 1}
+SCHED = {"auto": 0, "lds": 1, "split": 2}
 
 _dp = C.POINTER(C.c_double)
 
@@ -42,7 +43,7 @@ class T2mapOptsC(C.Structure):
         ("nRefAngles", C.c_int32), ("nRefAnglesMin", C.c_int32), ("Reg", C.c_int32), ("legacy", C.c_int32),
         ("Chi2Factor", C.c_double), ("NoiseLevel", C.c_double), ("RefConAngle", C.c_double),
         ("SetFlipAngle", C.c_double), ("SaveResidualNorm", C.c_int32), ("SaveDecayCurve", C.c_int32),
-        ("SaveRegParam", C.c_int32), ("SaveNNLSBasis", C.c_int32), ("precision", C.c_int32), ("_reserved", C.c_int32),
+        ("SaveRegParam", C.c_int32), ("SaveNNLSBasis", C.c_int32), ("precision", C.c_int32), ("schedule", C.c_int32),
     ]
 
 
@@ -69,7 +70,7 @@ class T2partOptsC(C.Structure):
 class StatsC(C.Structure):
     _fields_ = [("voxels", C.c_int64), ("nnls_solves", C.c_int64), ("fp64_flops", C.c_int64),
                 ("fp32_flops", C.c_int64), ("kernel_ms", C.c_double), ("total_ms", C.c_double),
-                ("voxels_failed", C.c_int64)]
+                ("voxels_failed", C.c_int64), ("lds_rows", C.c_int32), ("workgroups_per_cu", C.c_int32)]
 
 
 class ServerConfigC(C.Structure):

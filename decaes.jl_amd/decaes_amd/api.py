@@ -54,6 +54,7 @@ def t2map_opts_c(o: T2mapOptions) -> _lib.T2mapOptsC:
     c.SaveResidualNorm, c.SaveDecayCurve = int(o.SaveResidualNorm), int(o.SaveDecayCurve)
     c.SaveRegParam, c.SaveNNLSBasis = int(o.SaveRegParam), int(o.SaveNNLSBasis)
     c.precision = _lib.PREC[o.precision]
+    c.schedule = _lib.SCHED[o.schedule]
     return c
 
 

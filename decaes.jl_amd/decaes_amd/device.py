@@ -35,6 +35,14 @@ def alloc_t2map_outputs(opts: T2mapOptions, device="cuda"):
     return out
 
 
+def _ready(t):
+    """The library runs on its own HIP stream: work torch queued on its current stream for the arrays (an input
+    computed just before the call, outputs just filled) must be finished before the library reads them."""
+    import torch
+
+    torch.cuda.current_stream(t.device).synchronize()
+
+
 def _dptr(t):
     if t is None:
         return None
@@ -47,6 +55,7 @@ def t2map_device(image, opts: T2mapOptions, out: dict, alpha=None, ctx: Optional
     """dmi_t2map_f64_device on torch tensors (dmi_t2map_f64_device_noalloc with noalloc=True: no device allocation
     in the call); returns the call's device counters (dmi_last_stats)."""
     ctx = ctx or _lib.context(image.device.index or 0)
+    _ready(image)
     oc = t2map_opts_c(opts)
     oo = _lib.T2mapOutC(_dptr(out["dist"]), *(_dptr(out.get(k)) for k in (
         "gdn", "ggm", "gva", "fnr", "snr", "alpha", *_OPTIONAL)))
@@ -60,6 +69,7 @@ def t2map_device(image, opts: T2mapOptions, out: dict, alpha=None, ctx: Optional
 
 def t2part_device(dist, opts: T2partOptions, out: dict, ctx: Optional[_lib.Context] = None) -> None:
     ctx = ctx or _lib.context(dist.device.index or 0)
+    _ready(dist)
     oc = t2part_opts_c(opts)
     v = C.c_void_p
     rc = _lib.load().dmi_t2part_f64_device(ctx.handle, v(dist.data_ptr()), C.byref(oc), v(out["sfr"].data_ptr()),

@@ -57,6 +57,7 @@ class T2mapOptions:
     SaveNNLSBasis: bool = False
     Silent: bool = False
     precision: str = "fp64"
+    schedule: str = "auto"   # voxel-kernel launch schedule: "auto", "lds" or "split" (results are identical)
 
     def __post_init__(self):
         self.MatrixSize = tuple(int(v) for v in self.MatrixSize)
@@ -94,6 +95,7 @@ class T2mapOptions:
         assert self.SetFlipAngle is None or 0.0 <= self.SetFlipAngle <= 180.0, \
             f"Fixed flip angle must be in the range [0, 180], but SetFlipAngle = {self.SetFlipAngle}."
         assert self.precision in ("fp64", "mixed"), f"precision must be 'fp64' or 'mixed', got {self.precision!r}"
+        assert self.schedule in ("auto", "lds", "split"), f"schedule must be 'auto', 'lds' or 'split', got {self.schedule!r}"
 
     @classmethod
     def from_image(cls, image, **kwargs) -> "T2mapOptions":

@@ -34,6 +34,13 @@ enum { DMI_REG_NONE = 0, DMI_REG_LCURVE = 1, DMI_REG_GCV = 2, DMI_REG_CHI2 = 3, 
 /* Arithmetic of the voxel path: FP64 everywhere, or FP32 EPG bases with FP64 NNLS/regularization. */
 enum { DMI_PREC_FP64 = 0, DMI_PREC_MIXED = 1 };
 
+/* Launch schedule of the voxel kernel. LDS: the whole NNLS workspace of a voxel in LDS. SPLIT: the rows beyond a
+ * cutoff (Tikhonov rows appended late) in the workgroup's L2-resident global slot, so that more workgroups share a
+ * CU's LDS (Tikhonov-regularized methods, when it raises the workgroups per CU). AUTO: the choice measured on the
+ * first large call for these sizes and kept for the process (the all-LDS schedule until then). Every schedule
+ * performs the same arithmetic: results are bit-identical. */
+enum { DMI_SCHED_AUTO = 0, DMI_SCHED_LDS = 1, DMI_SCHED_SPLIT = 2 };
+
 /* T2mapOptions (src/types.jl:19-100). Optional reals use NaN for `nothing`. */
 typedef struct {
     int64_t MatrixSize[3];
@@ -57,7 +64,7 @@ typedef struct {
     int32_t SaveRegParam;
     int32_t SaveNNLSBasis;
     int32_t precision; /* DMI_PREC_* */
-    int32_t _reserved;
+    int32_t schedule;  /* DMI_SCHED_*: launch schedule of the voxel kernel (results do not depend on it) */
 } dmi_t2map_opts;
 
 /* T2Maps + T2Distributions fields (src/T2mapSEcorr.jl:2-19,64-72). Caller-allocated, column-major, and
@@ -90,6 +97,8 @@ typedef struct {
     double total_ms;       /* device time of the whole call */
     int64_t voxels_failed; /* voxels left unwritten (NaN) because a per-voxel search cache overflowed; the L-curve
                               caches hold every point the search can visit, so this stays 0 */
+    int32_t lds_rows;      /* schedule of the last launch: workspace rows held in LDS (the rest in the global slot) */
+    int32_t workgroups_per_cu; /* and its resident workgroups (one voxel each) per CU */
 } dmi_stats;
 
 /* Context: one HIP stream on `device`, reusable device buffers and cached grid bases.
@@ -104,7 +113,8 @@ const char *dmi_last_error(const dmi_ctx *ctx);
 int dmi_t2map_f64(dmi_ctx *ctx, const double *image, const dmi_t2map_opts *opts, const double *alpha_in,
                   const dmi_t2map_out *out);
 /* Same with every array already in device memory (hipMalloc'd on ctx's device); runs on ctx's stream and
- * returns after the stream has finished. */
+ * returns after the stream has finished. Work the caller queued on other streams for these arrays (an input still
+ * being written) must be complete before the call. */
 int dmi_t2map_f64_device(dmi_ctx *ctx, const double *d_image, const dmi_t2map_opts *opts, const double *d_alpha_in,
                          const dmi_t2map_out *d_out);
 

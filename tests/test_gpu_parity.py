@@ -292,3 +292,42 @@ def test_full_volume_sampled(dmi, oracle):
     s, stable = PU.check("c2_full_volume_sampled", oracle, T2R, sel, spread, maps, dist, omaps, odist)
     # rounding-level ties of the GCV search are rare
     assert (~stable).sum() <= 0.02 * stable.size, s
+
+
+@pytest.mark.parametrize("name,shape,m,n,K,reg,extra", [
+    ("c3_lcurve", (10, 10, 2), 56, 60, 64, "lcurve", {}),
+    ("c1_chi2", (12, 12, 2), 32, 40, 32, "chi2", dict(Chi2Factor=1.02)),
+    ("mdp", (10, 10, 2), 48, 40, 32, "mdp", dict(NoiseLevel=2e-3)),
+    ("chi2_m56_n60", (8, 8, 2), 56, 60, 64, "chi2", dict(Chi2Factor=1.02)),
+    ("mdp_m64_n60", (6, 6, 2), 64, 60, 32, "mdp", dict(NoiseLevel=2e-3)),
+    ("limits_lcurve", (3, 2, 1), 124, 64, 64, "lcurve", {}),
+    ("c2_gcv", (8, 8, 2), 48, 40, 32, "gcv", {}),  # no split schedule for GCV: "split" runs the all-LDS one
+])
+def test_split_schedule_is_bit_identical(dmi, name, shape, m, n, K, reg, extra):
+    """The split schedule (Tikhonov rows beyond an LDS cutoff kept in the workgroup's global slot, so that more
+    workgroups share a CU) performs the same arithmetic in the same order as the all-LDS schedule: every output is
+    bit-identical, and the split launch really holds fewer rows in LDS with more workgroups per CU."""
+    print(f"[schedule] case={name}", flush=True)
+    T2R = (8e-3, 2.0) if m == 56 else (10e-3, 2.0)
+    img, _ = dmi.synth_mse(shape, m, 7.9e-3 if m == 56 else 10e-3, ncomp=4 if m == 56 else 3, mask_fraction=1.0,
+                           seed=29)
+    res = {}
+    for sched in ("lds", "split"):
+        opts = dmi.T2mapOptions.from_image(img, TE=7.9e-3 if m == 56 else 10e-3, nT2=n, T2Range=T2R, Reg=reg,
+                                           nRefAngles=K, SaveRegParam=True, SaveResidualNorm=True, Silent=True,
+                                           schedule=sched, **extra)
+        maps, dist = dmi.T2mapSEcorr(img, opts)
+        st = dmi.api._lib.context(None).stats()
+        res[sched] = (maps, dist, st)
+    (ml, dl, sl), (ms, ds, ss) = res["lds"], res["split"]
+    print(f"[schedule] lds rows {sl['lds_rows']} x {sl['workgroups_per_cu']}/CU, "
+          f"split rows {ss['lds_rows']} x {ss['workgroups_per_cu']}/CU", flush=True)
+    assert np.array_equal(dl, ds, equal_nan=True)
+    for k in ("gdn", "ggm", "gva", "fnr", "snr", "alpha", "mu", "chi2factor", "resnorm"):
+        assert np.array_equal(ml[k], ms[k], equal_nan=True), k
+    assert sl["lds_rows"] == m + n
+    if reg == "gcv" or sl["workgroups_per_cu"] >= 4:
+        # no split for GCV; none either where the all-LDS schedule already has one workgroup per SIMD
+        assert ss["lds_rows"] == m + n and ss["workgroups_per_cu"] == sl["workgroups_per_cu"]
+    else:
+        assert ss["lds_rows"] < m + n and ss["workgroups_per_cu"] > sl["workgroups_per_cu"]
