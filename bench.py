@@ -168,6 +168,8 @@ def serve_levels(args) -> list:
                     if k < len(handles):
                         _, _, _, info = srv.wait(handles[k])
                         lat.append(info["latency_ms"])
+                        log(f"[serve] qps {qps}: request {k} done, {info['voxels']} voxels, "
+                            f"latency {info['latency_ms']:.0f} ms")
                         k += 1
                     else:
                         time.sleep(0.001)
