@@ -97,7 +97,6 @@ double erfinv(double y) {
 
 int validate_t2map(std::string &err, const dmi_t2map_opts *o) {
     if (!o) return errorf(err, DMI_EINVAL, "opts is NULL");
-    if (o->legacy) return errorf(err, DMI_EUNSUPPORTED, "legacy = true is not supported by the MI355X backend");
     if (!(o->MatrixSize[0] >= 1 && o->MatrixSize[1] >= 1 && o->MatrixSize[2] >= 1))
         return errorf(err, DMI_EINVAL, "MatrixSize must be a tuple of 3 positive integers, but MatrixSize = (%lld, %lld, %lld).",
                       (long long)o->MatrixSize[0], (long long)o->MatrixSize[1], (long long)o->MatrixSize[2]);
@@ -238,6 +237,7 @@ dmi::T2mapParams t2map_params(const dmi_t2map_opts *o, const GridPtrs &g, const 
     p.gld = 0;
     p.save_resnorm = o->SaveResidualNorm; p.save_decaycurve = o->SaveDecayCurve;
     p.save_regparam = o->SaveRegParam; p.save_basis = o->SaveNNLSBasis;
+    p.legacy = o->legacy != 0;
     p.V = o->MatrixSize[0] * o->MatrixSize[1] * o->MatrixSize[2];
     p.E1 = std::exp(-(o->TE / 2) / o->T1);
     p.beta = o->RefConAngle;
@@ -265,9 +265,9 @@ int t2map_launch_raw(std::string &err, dmi::T2mapParams p, const int64_t *d_vox,
                      unsigned long long *counters, int *status, double *slots, int64_t slot_doubles, int num_cu,
                      hipStream_t s) {
     if (nvox <= 0) return DMI_OK;
-    const size_t lds = dmi::t2map_lds_bytes(p.m, p.n, p.lda, p.K);
+    const size_t lds = dmi::t2map_lds_bytes(p.m, p.n, p.lda, p.K, p.legacy);
     int per_cu = 0;
-    DMI_HIP(err, dmi::t2map_occupancy(p.reg, &per_cu, lds));
+    DMI_HIP(err, dmi::t2map_occupancy(p.reg, p.legacy, &per_cu, lds));
     if (per_cu < 1) return errorf(err, DMI_EUNSUPPORTED, "voxel kernel does not fit on a CU (LDS %zu bytes)", lds);
     p.slot_stride = dmi::t2map_slot_doubles(p.m, p.n, p.gld);
     const int64_t fit = slot_doubles / p.slot_stride;
@@ -291,9 +291,9 @@ int t2map_launch(std::string &err, dmi::T2mapParams p, const int64_t *d_vox, int
     DMI_HIP(err, lb.counter.ensure(sizeof(unsigned) * 4));
     DMI_HIP(err, lb.counters.ensure(sizeof(unsigned long long) * 4));
     DMI_HIP(err, lb.status.ensure(sizeof(int) * 4));
-    const size_t lds = dmi::t2map_lds_bytes(p.m, p.n, p.lda, p.K);
+    const size_t lds = dmi::t2map_lds_bytes(p.m, p.n, p.lda, p.K, p.legacy);
     int per_cu = 0;
-    DMI_HIP(err, dmi::t2map_occupancy(p.reg, &per_cu, lds));
+    DMI_HIP(err, dmi::t2map_occupancy(p.reg, p.legacy, &per_cu, lds));
     if (per_cu < 1) return errorf(err, DMI_EUNSUPPORTED, "voxel kernel does not fit on a CU (LDS %zu bytes)", lds);
     const int64_t grid = std::min<int64_t>((int64_t)num_cu * per_cu, nvox);
     const int64_t slot_doubles = dmi::t2map_slot_doubles(p.m, p.n, p.gld) * grid;
@@ -309,7 +309,7 @@ int t2map_schedules(std::string &err, const dmi_t2map_opts *o, std::vector<Sched
     full.lda = dmi::t2map_lda(m, n);
     full.rl = full.lda;
     full.gld = 0;
-    DMI_HIP(err, dmi::t2map_occupancy(o->Reg, &full.per_cu, dmi::t2map_lds_bytes(m, n, full.lda, K)));
+    DMI_HIP(err, dmi::t2map_occupancy(o->Reg, o->legacy, &full.per_cu, dmi::t2map_lds_bytes(m, n, full.lda, K, o->legacy)));
     out.push_back(full);
     // Only the Tikhonov-regularized solves append rows beyond m; GCV also keeps its Gram matrix in rows [m, m + n)
     if (o->Reg != DMI_REG_LCURVE && o->Reg != DMI_REG_CHI2 && o->Reg != DMI_REG_MDP) return DMI_OK;
@@ -320,7 +320,7 @@ int t2map_schedules(std::string &err, const dmi_t2map_opts *o, std::vector<Sched
     for (int target = full.per_cu + 1; target <= full.per_cu + 2; ++target) {
         for (int lda = full.lda - 4; lda >= rmin; lda -= 4) {
             int pc = 0;
-            DMI_HIP(err, dmi::t2map_occupancy(o->Reg, &pc, dmi::t2map_lds_bytes(m, n, lda, K)));
+            DMI_HIP(err, dmi::t2map_occupancy(o->Reg, o->legacy, &pc, dmi::t2map_lds_bytes(m, n, lda, K, o->legacy)));
             if (pc >= target) {
                 if (pc > best) {
                     Schedule s;
@@ -379,7 +379,7 @@ void apply_schedule(dmi::T2mapParams &p, const Schedule &s) {
 int t2part_prepare(std::string &err, const dmi_t2part_opts *o, std::vector<double> &lg, std::vector<double> &wts,
                    dmi::T2partParams &p) {
     if (!o) return errorf(err, DMI_EINVAL, "opts is NULL");
-    if (o->legacy) return errorf(err, DMI_EUNSUPPORTED, "legacy = true is not supported by the MI355X backend");
+    // legacy changes nothing in T2partSEcorr (src/T2partSEcorr.jl never reads it)
     if (!(o->MatrixSize[0] >= 1 && o->MatrixSize[1] >= 1 && o->MatrixSize[2] >= 1)) return errorf(err, DMI_EINVAL, "all(MatrixSize .>= 1)");
     if (!(o->nT2 >= 2)) return errorf(err, DMI_EINVAL, "nT2 >= 2");
     if (!(0.0 < o->T2Range[0] && o->T2Range[0] < o->T2Range[1])) return errorf(err, DMI_EINVAL, "0.0 < T2Range[1] < T2Range[2]");

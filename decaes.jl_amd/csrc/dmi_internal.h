@@ -12,6 +12,13 @@ constexpr int kMaxEcho = 124;     // lane-per-state EPG window (ETL/2 + 2 states
 constexpr int kMaxAngles = 64;    // one lane per flip-angle grid point
 constexpr int kTikhCache = 8;     // NNLSTikhonovRegProblemCache size (src/lsqnonneg.jl:396)
 constexpr int kLcurveCap = 32;    // L-curve cache entries per voxel: at most 28 are used (4 + 24 bracket moves)
+// legacy = true: interpolating splines of at most 64 points (the flip-angle grid; the chi2 doubling search has at
+// most 2 + kLegacyMaxDoublings), knots and coefficients in LDS
+constexpr int kSplineMaxPts = 64;
+constexpr int kSplineKnots = kSplineMaxPts + 8;                 // m + k + 1 <= 68, rounded up to keep 16-byte alignment
+constexpr int kSplineLds = kSplineKnots + kSplineMaxPts;        // doubles
+constexpr int kLegacyMaxDoublings = 24;                         // chi2 legacy: mu <= 1e-3 * 2^24 (2^24 + 1 samples)
+constexpr long long kSplineMaxSamples = (1ll << 24) + 1;        // longest sample grid a legacy scan walks
 
 // Column stride of the column-major bases in device memory (grid bases, fixed basis, the voxel's basis): m rounded
 // up to even, so every column starts 16-byte aligned and is copied into LDS with 16-byte global-to-LDS loads.
@@ -57,6 +64,7 @@ struct T2mapParams {
     int lda, mw;          // workspace: mw = m + n rows, column-major; rows [0, rl) in LDS with column stride lda
     int rl, gld;          // rows [rl, mw) in the workgroup's global slot, column stride gld (rl >= mw: all in LDS)
     int save_resnorm, save_decaycurve, save_regparam, save_basis;
+    int legacy;           // legacy = true: spline surrogate for the flip angle, chi2 doubling search
     int64_t V;            // voxels in the volume
     int64_t nvox;         // voxels above threshold
     double E1, beta, chi2factor, noiselevel, setflip;
@@ -88,7 +96,8 @@ struct T2partParams {
 };
 
 int t2map_lda(int m, int n);
-size_t t2map_lds_bytes(int m, int n, int lda, int K);  // K: flip-angle grid size of the launch (1 with SetFlipAngle)
+// K: flip-angle grid size of the launch (1 with SetFlipAngle); legacy adds the spline knots and coefficients
+size_t t2map_lds_bytes(int m, int n, int lda, int K, int legacy);
 int64_t t2map_slot_doubles(int m, int n, int gld);
 
 hipError_t launch_grid_basis(const GridParams &p, hipStream_t s);
@@ -105,6 +114,6 @@ hipError_t launch_t2part(const T2partParams &p, hipStream_t s);
 // Float32 <-> Float64 element-wise conversion (Float32 entry points)
 hipError_t launch_widen(const float *src, double *dst, int64_t n, hipStream_t s);
 hipError_t launch_narrow(const double *src, float *dst, int64_t n, hipStream_t s);
-hipError_t t2map_occupancy(int reg, int *blocks_per_cu, size_t lds);
+hipError_t t2map_occupancy(int reg, int legacy, int *blocks_per_cu, size_t lds);
 
 }  // namespace dmi

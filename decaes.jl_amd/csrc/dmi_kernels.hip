@@ -545,6 +545,7 @@ struct Wave {
     // a lane that owns a column streams it with 16-byte loads, and 16-byte loads of 16 consecutive lanes
     // at stride lda hit distinct banks. Every LDS double array starts 16-byte aligned.
     ldbl *A, *b, *zz, *x, *xf, *b0, *cmu, *crn, *csemi, *cx;
+    ldbl *spt, *spc;  // legacy = true: spline knots (kSplineKnots) and coefficients (kSplineMaxPts)
     lint *idx, *diag, *stk;
     int lda, mw, m, n, L;
     // Rows [0, rl) of the workspace are in LDS (column stride lda); rows [rl, mw) of column j, when rl < mw, are in
@@ -1333,9 +1334,243 @@ template <class F> __device__ __forceinline__ void brent_minimize(F &&f, double 
     yo = y;
 }
 
+// ------------------------------------ legacy path (legacy = true) ------------------------------------
+// Dierckx interpolating splines sampled on a 0.001 grid (src/splines.jl:312-315, 414-446, 494-500) and the chi2
+// doubling search (src/lsqnonneg.jl:521-532, 595-636). Dierckx.jl wraps P. Dierckx's FITPACK: curfit with s = 0
+// (the knots of fpcurf's interpolation case, Givens rotations fpgivs / fprota of each collocation row into a band
+// triangle, back substitution fpback) and splev (the de Boor-Cox values of fpbspl). Each step keeps FITPACK's order
+// of plain multiplies and adds (its Fortran is built for generic x86-64, without fma), so values match FITPACK's.
+
+// Julia's Float64 range start:step:stop (base/twiceprecision.jl). With small exact rationals for start, step and
+// stop (rat: continued fractions bounded by maxintfloat(Float32)), element i is (sn + i stn) / den to twice
+// precision, i.e. the correctly rounded quotient of exact integers; otherwise start + i*step, product and sum each
+// rounded.
+struct JlRange {
+    bool rational;
+    long long sn, stn, den, len;
+    double start, step;
+    __device__ __forceinline__ double at(long long i) const {
+        return rational ? (double)(sn + i * stn) / (double)den : start + (double)i * step;
+    }
+};
+__device__ __forceinline__ void jl_rat(double x, long long &num, long long &den) {
+    const double mf = 16777216.0;
+    double y = x;
+    long long a = 1, d = 1, b = 0, c = 0;
+    while (fabs(y) <= mf) {
+        const long long f = (long long)trunc(y);
+        y -= (double)f;
+        const long long a2 = f * a + c, b2 = f * b + d;
+        c = a; a = a2;
+        d = b; b = b2;
+        if (!(llabs(a) <= 16777216ll && llabs(b) <= 16777216ll)) { num = c; den = d; return; }
+        if ((double)a / (double)b == x) break;
+        y = 1.0 / y;
+    }
+    num = a; den = b;
+}
+__device__ __forceinline__ bool jl_isbetween(double a, double x, double b) {
+    return (a <= x && x <= b) || (b <= x && x <= a);
+}
+__device__ JlRange jl_range(double start, double step, double stop) {
+    JlRange r{false, 0, 0, 1, 0, start, step};
+    long long step_n, step_d;
+    jl_rat(step, step_n, step_d);
+    if (step_d != 0 && (double)step_n / (double)step_d == step) {
+        long long start_n, start_d, stop_n, stop_d;
+        jl_rat(start, start_n, start_d);
+        jl_rat(stop, stop_n, stop_d);
+        if (start_d != 0 && stop_d != 0 && (double)start_n / (double)start_d == start &&
+            (double)stop_n / (double)stop_d == stop) {
+            long long g = llabs(start_d), h = llabs(step_d);
+            while (h) { const long long t = g % h; g = h; h = t; }
+            const long long den = llabs(start_d / g * step_d);
+            const double mi = 9007199254740992.0;
+            if (den != 0 && fabs(start * (double)den) <= mi && fabs(step * (double)den) <= mi && den % start_d == 0 &&
+                den % step_d == 0) {
+                const long long sn = llrint(start * (double)den), stn = llrint(step * (double)den);
+                // Int64 arithmetic wraps in Julia; unsigned arithmetic wraps the same way here
+                typedef unsigned long long u64;
+                const long long num = (long long)((u64)den * (u64)stop_n - (u64)stop_d * (u64)sn + (u64)stn * (u64)stop_d);
+                long long len = num / (long long)((u64)stn * (u64)stop_d);
+                if (len < 0) len = 0;
+                if (jl_isbetween(start, start + (double)(len - 1) * step, stop + step / 2) &&
+                    !jl_isbetween(start, start + (double)len * step, stop)) {
+                    r.rational = true; r.sn = sn; r.stn = stn; r.den = den; r.len = len;
+                    return r;
+                }
+            }
+        }
+    }
+    const double lf = (stop - start) / step;
+    long long len;
+    if (lf < 0) len = 0;
+    else if (lf == 0) len = 1;
+    else {
+        len = llrint(lf) + 1;
+        const double stop2 = start + (double)(len - 1) * step;
+        len -= (long long)(start < stop && stop < stop2) + (long long)(start > stop && stop > stop2);
+    }
+    r.len = len;
+    return r;
+}
+
+// fpbspl: h[1..k+1] = the nonzero B-splines of degree k <= 3 at x, t(l) <= x < t(l + 1); t1 is 1-based
+__device__ __forceinline__ void fpbspl(const ldbl *t1, int k, double x, int l, double h[5]) {
+    double hh[3];
+    h[1] = 1.0;
+#pragma unroll
+    for (int j = 1; j <= 3; ++j) {
+        if (j > k) break;
+#pragma unroll
+        for (int i = 1; i <= j; ++i) hh[i - 1] = h[i];
+        h[1] = 0.0;
+#pragma unroll
+        for (int i = 1; i <= j; ++i) {
+            const int li = l + i, lj = li - j;
+            const double tli = t1[li], tlj = t1[lj];
+            if (tli == tlj) { h[i + 1] = 0.0; continue; }
+            const double f = hh[i - 1] / (tli - tlj);
+            h[i] = h[i] + f * (tli - x);
+            h[i + 1] = f * (x - tlj);
+        }
+    }
+}
+__device__ __forceinline__ void fpgivs(double piv, double &ww, double &cs, double &sn) {
+    const double store = fabs(piv);
+    double dd;
+    if (store >= ww) dd = store * sqrt(1.0 + (ww / piv) * (ww / piv));
+    else dd = ww * sqrt(1.0 + (piv / ww) * (piv / ww));
+    cs = ww / dd;
+    sn = piv / dd;
+    ww = dd;
+}
+__device__ __forceinline__ void fprota(double cs, double sn, double &a, double &b) {
+    const double s1 = a, s2 = b;
+    b = cs * s2 + sn * s1;
+    a = cs * s1 - sn * s2;
+}
+
+// Interpolating spline of degree k = min(3, np - 1) through the np (2 <= np <= 64) points held one per lane (lane i:
+// point i + 1, x increasing). Knots go to W.spt, coefficients to W.spc; returns the knot count n = np + k + 1.
+// Lane j - 1 holds row j of the band triangle (4 entries) and of the rotated right-hand side; every rotation reads
+// its row with readlane, computes in all lanes alike, and the owner keeps the result.
+__device__ int spline_fit(Wave &W, double xv, double yv, int np, int &k_out) {
+    const int L = W.L;
+    const int k = min(3, np - 1), k1 = k + 1, k2 = k + 2, n = np + k1, nk1 = np, k3 = k / 2;
+    ldbl *t1 = W.spt - 1;
+    // knots: t(1..k1) = x(1), t(n-k..n) = x(np); t(i) for k2 <= i <= n - k1: x(j) (odd k) or (x(j) + x(j-1)) / 2
+    // (even k), j = i - k + k3
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int i = L + 1 + r * kWave;
+        const int j = min(max(i - k + k3, 2), np);
+        const double xj = __shfl(xv, j - 1, kWave), xj1 = __shfl(xv, j - 2, kWave);
+        const double x1 = readlane(xv, 0), xe = readlane(xv, np - 1);
+        double v = (k3 * 2 == k) ? (xj + xj1) * 0.5 : xj;
+        if (i <= k1) v = x1;
+        if (i > n - k1) v = xe;
+        if (i <= n) t1[i] = v;
+    }
+    lds_order();
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, zr = 0.0;
+    int l = k1;
+    for (int it = 1; it <= np; ++it) {
+        const double xi = readlane(xv, it - 1);
+        double yi = readlane(yv, it - 1);
+        while (!(xi < rfl(t1[l + 1]) || l == nk1)) ++l;
+        double h[5];
+        fpbspl(t1, k, xi, l, h);
+        int jj = l - k1;
+#pragma unroll
+        for (int ii = 1; ii <= 4; ++ii) {
+            if (ii > k1) break;
+            ++jj;
+            const double piv = h[ii];
+            if (piv == 0.0) continue;
+            double r[4] = {readlane(a0, jj - 1), readlane(a1, jj - 1), readlane(a2, jj - 1), readlane(a3, jj - 1)};
+            double zj = readlane(zr, jj - 1), cs, sn;
+            fpgivs(piv, r[0], cs, sn);
+            fprota(cs, sn, yi, zj);
+#pragma unroll
+            for (int i1 = ii + 1; i1 <= 4; ++i1) {
+                if (i1 > k1) break;
+                fprota(cs, sn, h[i1], r[i1 - ii]);
+            }
+            if (L == jj - 1) { a0 = r[0]; a1 = r[1]; a2 = r[2]; a3 = r[3]; zr = zj; }
+        }
+    }
+    // fpback: c(i) = (z(i) - sum_{q=1..min(k, nk1-i)} c(i+q) a(i, q+1)) / a(i, 1), from i = nk1 down
+    ldbl *c1p = W.spc - 1;
+    double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+    for (int i = nk1; i >= 1; --i) {
+        double store = readlane(zr, i - 1);
+        const double r0 = readlane(a0, i - 1), r1 = readlane(a1, i - 1), r2 = readlane(a2, i - 1), r3 = readlane(a3, i - 1);
+        const int i1 = min(k, nk1 - i);
+        if (i1 >= 1) store = store - c1 * r1;
+        if (i1 >= 2) store = store - c2 * r2;
+        if (i1 >= 3) store = store - c3 * r3;
+        const double ci = store / r0;
+        if (L == 0) c1p[i] = ci;
+        c3 = c2; c2 = c1; c1 = ci;
+    }
+    lds_order();
+    W.flops += 40ull * np * k1 + 8ull * np;
+    k_out = k;
+    return n;
+}
+
+// spline_opt_legacy (src/splines.jl:419-431; root = false) / spline_root_legacy (:435-446; root = true, the value
+// |s(x) - v|) over knots[1]:0.001:knots[end] (knots = t(k+1..n-k)): the first sample of least value wins, as the
+// reference's scan with strict '<' keeps it (a NaN first value is never replaced). Lane L scans a contiguous block
+// of samples; the interval search starts where the lane's previous sample left it (the samples increase, so this
+// is the interval splev finds from t(k+1)).
+__device__ void spline_scan(Wave &W, int k, int n, bool root, double v, double &xo, double &yo) {
+    const int L = W.L, k1 = k + 1, nk1 = n - k1;
+    const ldbl *t1 = W.spt - 1, *c1p = W.spc - 1;
+    const JlRange r = jl_range(rfl(t1[k1]), 0.001, rfl(t1[n - k]));
+    const long long len = r.len;
+    if (!(len >= 1 && len <= kSplineMaxSamples)) {  // only non-finite knots get here; the voxel gets NaN
+        W.failed = true;
+        xo = NAN; yo = NAN;
+        return;
+    }
+    const long long per = (len + kWave - 1) / kWave, i0 = (long long)L * per, i1 = min(len, i0 + per);
+    double best = INFINITY, first = NAN;
+    long long bi = -1;
+    int l = k1;
+    for (long long i = i0; i < i1; ++i) {
+        const double xi = r.at(i);
+        while (!(xi < t1[l + 1] || l == nk1)) ++l;
+        double h[5];
+        fpbspl(t1, k, xi, l, h);
+        double sp = 0.0;
+        int ll = l - k1;
+#pragma unroll
+        for (int j = 1; j <= 4; ++j) {
+            if (j > k1) break;
+            ++ll;
+            sp = sp + c1p[ll] * h[j];
+        }
+        const double y = root ? fabs(sp - v) : sp;
+        if (i == 0) first = y;
+        if (!isnan(y) && (bi < 0 || y < best)) { best = y; bi = i; }
+    }
+    W.flops += (unsigned long long)len * (unsigned long long)(6 * k * (k + 1) / 2 + 2 * k1 + 2);
+    const double y0 = readlane(first, 0);
+    int ord = bi >= 0 ? L : INT_MAX;
+    if (bi < 0) best = INFINITY;
+    wargmin(best, ord);
+    if (isnan(y0) || ord == INT_MAX) { xo = r.at(0); yo = y0; return; }
+    const long long b = (long long)__shfl((long long)bi, ord, kWave);
+    xo = r.at(b);
+    yo = best;
+}
+
 // ------------------------------------ regularization (src/lsqnonneg.jl) ------------------------------------
 // lsqnonneg_chi2! with method :brent (:504-593)
-__device__ __forceinline__ void reg_chi2(Wave &W, double chi2_target, double &mu_out, double &chi2_out) {
+// (legacy: method :legacy, :521-532, with chi2_search_from_minimum)
+__device__ __forceinline__ void reg_chi2(Wave &W, double chi2_target, bool legacy, double &mu_out, double &chi2_out) {
     int ns;
     double res2_min = W.plain_solve_res2(ns);
     if (res2_min == 0 || ns == 0) {
@@ -1348,6 +1583,36 @@ __device__ __forceinline__ void reg_chi2(Wave &W, double chi2_target, double &mu
     __syncthreads();
     const double target = chi2_target * res2_min;
     W.cache_reset();
+    if (legacy) {
+        // chi2_search_from_minimum (:595-636): lane i holds the (mu, res2) pair i, starting from (0, res2_min)
+        double mu_l = (W.L == 0) ? 0.0 : NAN, r2_l = (W.L == 0) ? res2_min : NAN;
+        int np = 1;
+        double munew = 1e-3;
+        for (int it = 0;; ++it) {
+            W.cache_solve(munew);
+            const double r2 = W.cur_resnorm_sq();
+            if (W.L == np) { mu_l = munew; r2_l = r2; }
+            np += 1;
+            if (r2 >= chi2_target * res2_min) break;
+            if (it == kLegacyMaxDoublings) { W.failed = true; return; }  // the reference keeps doubling without end
+            munew *= 2.0;
+        }
+        int k;
+        const int nk = spline_fit(W, mu_l, r2_l, np, k);
+        double mu, unused;
+        spline_scan(W, k, nk, true, chi2_target * res2_min, mu, unused);
+        double res2;
+        if (mu == 0) {
+            res2 = res2_min;  // x_final = x_unreg, already in xf
+        } else {
+            W.cache_solve(mu);
+            res2 = W.cur_resnorm_sq();
+            W.take_cur_solution();
+        }
+        mu_out = mu;
+        chi2_out = res2 / res2_min;
+        return;
+    }
     auto f = [&](double logmu) {
         W.cache_solve(exp(logmu));
         return (W.cur_resnorm_sq() - target) / target;
@@ -1980,6 +2245,9 @@ struct FlipSearch {
     int K, numeval, mineval;
     unsigned long long seen;
     double al, u, du;  // lane k: grid angle, loss, gradient
+    bool legacy;       // CubicSplineSurrogate (loss only, spline_opt_legacy) instead of the Hermite surrogate
+    unsigned long long lseen;  // legacy: the evaluated set of the last suggestion, and that suggestion
+    double lx, lu;
 
     __device__ __forceinline__ void eval(int k) {
         const int m = W.m, n = W.n;
@@ -1989,6 +2257,10 @@ struct FlipSearch {
         double rn;
         W.solve(AT, false, 0.0, ns, rn);
         double ell = rn * rn;
+        if (legacy) {  // loss! (src/splines.jl:1010-1014): no gradient
+            if (W.L == k) { u = ell; du = 0.0; }
+            return;
+        }
         double part = 0.0;
         for (int i = W.L; i < m; i += kWave) {
             double ax = 0.0, dax = 0.0;
@@ -2041,8 +2313,25 @@ struct FlipSearch {
         if (box_evaluated(lo, hi)) return;
         update(c1);
     }
+    // suggest_point of CubicSplineSurrogate (:494-500): spline_opt_legacy through the evaluated points in grid
+    // order. The result depends only on the evaluated set, so an unchanged set reuses the last one.
+    __device__ void suggest_legacy(double &xo, double &uo) {
+        if (seen == lseen) { xo = lx; uo = lu; return; }
+        const int L = W.L;
+        const bool mine = (seen >> L) & 1ull;
+        const int rank = __popcll(seen & ((1ull << L) - 1ull)), np = __popcll(seen);
+        if (mine) { W.spt[rank] = al; W.spc[rank] = u; }
+        lds_order();
+        const double xv = (L < np) ? (double)W.spt[L] : 0.0, yv = (L < np) ? (double)W.spc[L] : 0.0;
+        lds_order();
+        int k;
+        const int nk = spline_fit(W, xv, yv, np, k);
+        spline_scan(W, k, nk, false, 0.0, xo, uo);
+        lseen = seen; lx = xo; lu = uo;
+    }
     // suggest_point (:544-566): one Hermite segment per evaluated lane, earliest minimum wins.
     __device__ __forceinline__ void suggest(double &xo, double &uo) {
+        if (legacy) { suggest_legacy(xo, uo); return; }
         const int k = W.L;
         int f = first_set(seen);
         double uf = readlane(u, f), xf = readlane(al, f);
@@ -2167,8 +2456,12 @@ __device__ __forceinline__ void voxel_basis(Wave &W, bool fp32, bool general, do
 }
 
 // ------------------------------------ the voxel kernel ------------------------------------
-template <int REG>
+// RK = Reg (0-4) + 8 for legacy = true: the legacy code is compiled only into the legacy instantiations, so the
+// others keep their register allocation.
+template <int RK>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void t2map_kernel(T2mapParams p) {
+    constexpr int REG = RK & 7;
+    constexpr bool LEGACY = (RK & 8) != 0;
     extern __shared__ double smem[];
     Wave W;
     W.L = lane_id();
@@ -2192,6 +2485,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     W.crn = s; s += kTikhCache;
     W.csemi = s; s += kTikhCache;
     W.cx = s; s += (size_t)kTikhCache * p.n;  // the cached Tikhonov solutions, n per entry
+    W.spt = s; s += LEGACY ? kSplineKnots : 0;
+    W.spc = s; s += LEGACY ? kSplineMaxPts : 0;
     lint *si = (lint *)s;
     W.idx = si; si += ne;
     W.diag = si; si += ne;
@@ -2217,6 +2512,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
         vi = (unsigned)rfl((int)__shfl((int)vi, 0, kWave));
         if ((int64_t)vi >= p.nvox) break;
         const int64_t I = ((cgi64 *)p.vox)[vi];
+        W.failed = false;
         // voxelwise_T2_distribution! (:201-238): normalize by the max echo
         double mx = 0.0;
         for (int i = L; i < m; i += kWave) {
@@ -2237,7 +2533,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
             if (p.mode == kModeB1) {
                 alpha_v = p.alpha_in[I];
             } else {
-                FlipSearch fs{W, (cgdbl *)p.AsT, (cgdbl *)p.dAsT, p.K, 0, p.mineval, 0ull, 0.0, 0.0, 0.0};
+                FlipSearch fs{W, (cgdbl *)p.AsT, (cgdbl *)p.dAsT, p.K, 0, p.mineval, 0ull, 0.0, 0.0, 0.0,
+                              LEGACY, 0ull, 0.0, 0.0};
                 fs.al = (L < p.K) ? p.alphas[L] : p.alphas[p.K - 1];
                 fs.u = NAN;
                 fs.du = NAN;
@@ -2252,12 +2549,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
         }
         // T2_distribution! (src/T2mapSEcorr.jl:475-505)
         double muv = 0.0, chi2v = 1.0;
-        W.failed = false;
         DMI_PT0();
         if constexpr (REG == 0) W.plain_solve_to_xf();
         else if constexpr (REG == 1) reg_lcurve(W, muv, chi2v);
         else if constexpr (REG == 2) reg_gcv(W, muv, chi2v);
-        else if constexpr (REG == 3) reg_chi2(W, p.chi2factor, muv, chi2v);
+        else if constexpr (REG == 3) reg_chi2(W, p.chi2factor, LEGACY, muv, chi2v);
         else {
             double sigma = p.noiselevel / max_signal;
             double delta = sqrt((double)m) * sigma;
@@ -2378,10 +2674,10 @@ int t2map_lda(int m, int n) {
     return (ld % 4 == 0) ? ld + 2 : ld;
 }
 
-size_t t2map_lds_bytes(int m, int n, int lda, int K) {
+size_t t2map_lds_bytes(int m, int n, int lda, int K, int legacy) {
     const size_t ne = (size_t)((n + 1) & ~1), bz = (size_t)((m + n + 1) & ~1);
     size_t d = (size_t)n * lda + 2 * bz + 2 * ne + (size_t)((m + 1) & ~1) + 3 * kTikhCache +
-               (size_t)kTikhCache * n;
+               (size_t)kTikhCache * n + (legacy ? (size_t)kSplineLds : 0);
     size_t i = 2 * ne + 3 * ((size_t)K + 2);
     return d * sizeof(double) + i * sizeof(int);
 }
@@ -2439,7 +2735,16 @@ hipError_t launch_select(const double *image, int64_t v0, int64_t v1, double thr
 }
 
 using T2mapKernelFn = void (*)(T2mapParams);
-static T2mapKernelFn t2map_kernel_for(int reg) {
+static T2mapKernelFn t2map_kernel_for(int reg, int legacy) {
+    if (legacy) {
+        switch (reg) {
+        case 0: return t2map_kernel<8>;
+        case 1: return t2map_kernel<9>;
+        case 2: return t2map_kernel<10>;
+        case 3: return t2map_kernel<11>;
+        default: return t2map_kernel<12>;
+        }
+    }
     switch (reg) {
     case 0: return t2map_kernel<0>;
     case 1: return t2map_kernel<1>;
@@ -2450,12 +2755,12 @@ static T2mapKernelFn t2map_kernel_for(int reg) {
 }
 
 hipError_t launch_t2map(const T2mapParams &p, int grid, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL(t2map_kernel_for(p.reg), dim3(grid), dim3(64), lds, s, p);
+    hipLaunchKernelGGL(t2map_kernel_for(p.reg, p.legacy), dim3(grid), dim3(64), lds, s, p);
     return hipGetLastError();
 }
 
-hipError_t t2map_occupancy(int reg, int *blocks_per_cu, size_t lds) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, t2map_kernel_for(reg), 64, lds);
+hipError_t t2map_occupancy(int reg, int legacy, int *blocks_per_cu, size_t lds) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, t2map_kernel_for(reg, legacy), 64, lds);
 }
 
 hipError_t launch_fill(void *p, size_t bytes, unsigned int v, hipStream_t s) {

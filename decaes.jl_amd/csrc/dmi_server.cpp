@@ -250,11 +250,11 @@ int bind_and_select(dmi_server *S, StreamState &st, Request *r) {
     // while a chunk runs
     const int m = r->o.nTE, n = r->o.nT2;
     const int lda = dmi::t2map_lda(m, n);
-    const size_t lds = dmi::t2map_lds_bytes(m, n, lda, fixed ? 1 : r->o.nRefAngles);  // as t2map_params sets K
+    const size_t lds = dmi::t2map_lds_bytes(m, n, lda, fixed ? 1 : r->o.nRefAngles, r->o.legacy);  // as t2map_params sets K
     int grid = 0;
     for (int reg = DMI_REG_NONE; reg <= DMI_REG_MDP; ++reg) {
         int per_cu = 0;
-        DMI_HIP(e, dmi::t2map_occupancy(reg, &per_cu, lds));
+        DMI_HIP(e, dmi::t2map_occupancy(reg, r->o.legacy, &per_cu, lds));
         grid = std::max(grid, per_cu * S->num_cu);
     }
     // the request's launch schedule (its explicit choice, or the one measured for these sizes, or all-LDS)

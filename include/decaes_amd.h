@@ -21,7 +21,7 @@ typedef struct dmi_ctx dmi_ctx;
 enum {
     DMI_OK = 0,
     DMI_EINVAL = -1,       /* an option fails the reference's @assert (message says which) */
-    DMI_EUNSUPPORTED = -2, /* legacy=true, nT2 > 64 or nTE > 124: not handled by this backend */
+    DMI_EUNSUPPORTED = -2, /* nT2 > 64 or nTE > 124: beyond the device limits */
     DMI_ENOMEM = -3,
     DMI_EHIP = -4,         /* HIP runtime error */
     DMI_ENODEV = -5,       /* no usable gfx950 device */
@@ -54,7 +54,7 @@ typedef struct {
     int32_t nRefAngles;
     int32_t nRefAnglesMin;
     int32_t Reg;
-    int32_t legacy;
+    int32_t legacy;   /* spline flip-angle surrogate and chi2 doubling search (src/lsqnonneg.jl:595-636) */
     double Chi2Factor;
     double NoiseLevel;
     double RefConAngle;

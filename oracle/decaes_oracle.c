@@ -1508,6 +1508,263 @@ static void lsq_none(reg_ws *R, double *x_out, double *mu_out, double *chi2_out)
     *mu_out = 0; *chi2_out = 1;
 }
 
+/* ------------------------------------------------------------------------------------------------
+ * Legacy path (legacy = true): Dierckx interpolating splines sampled on a 0.001 grid
+ * (src/splines.jl:312-315, 414-446, 498) and the chi2 doubling search (src/lsqnonneg.jl:521-532, 595-636).
+ *
+ * Dierckx.Spline1D(x, y; k, s = 0) is the third-party package Dierckx.jl (Project.toml: Dierckx), a wrapper of
+ * P. Dierckx's FITPACK (netlib): curfit with s = 0 gives the interpolating spline with the knots of fpcurf's
+ * interpolation case, the coefficients by Givens rotations of the collocation rows (fpgivs, fprota) and back
+ * substitution (fpback); values by the de Boor-Cox recursion (fpbspl) in splev. The restatement below follows
+ * that published algorithm step by step, without fma, as FITPACK's Fortran is compiled for generic x86-64.
+ * ------------------------------------------------------------------------------------------------ */
+
+/* fpbspl: the k + 1 nonzero B-splines at x, t(l) <= x < t(l + 1); t and h are 1-based */
+static void fpbspl(const double *t, int k, double x, int l, double *h) {
+    double hh[8];
+    h[1] = 1.0;
+    for (int j = 1; j <= k; ++j) {
+        for (int i = 1; i <= j; ++i) hh[i] = h[i];
+        h[1] = 0.0;
+        for (int i = 1; i <= j; ++i) {
+            int li = l + i, lj = li - j;
+            if (t[li] == t[lj]) { h[i + 1] = 0.0; continue; }
+            double f = hh[i] / (t[li] - t[lj]);
+            h[i] = h[i] + f * (t[li] - x);
+            h[i + 1] = f * (x - t[lj]);
+        }
+    }
+}
+/* fpgivs: Givens parameters that rotate piv into ww (ww becomes the new pivot) */
+static void fpgivs(double piv, double *ww, double *cs, double *sn) {
+    double store = fabs(piv), dd;
+    if (store >= *ww) dd = store * sqrt(1.0 + (*ww / piv) * (*ww / piv));
+    else dd = *ww * sqrt(1.0 + (piv / *ww) * (piv / *ww));
+    *cs = *ww / dd;
+    *sn = piv / dd;
+    *ww = dd;
+}
+/* fprota: (a, b) <- (cs a - sn b, cs b + sn a) */
+static void fprota(double cs, double sn, double *a, double *b) {
+    double s1 = *a, s2 = *b;
+    *b = cs * s2 + sn * s1;
+    *a = cs * s1 - sn * s2;
+}
+
+/* Interpolating spline of degree k through (x[i], y[i]), i < m, x increasing, 1 <= k <= 5, m > k. Writes the
+ * n = m + k + 1 knots to t and the m coefficients to c (both 0-based); returns n. */
+int orc_spline_interp(const double *x0, const double *y0, int m, int k, double *t0, double *c0) {
+    if (k < 1 || k > 5 || m <= k) return -1;
+    const int k1 = k + 1, k2 = k + 2, n = m + k1, nk1 = n - k1;
+    const double *x = x0 - 1, *y = y0 - 1;
+    double *t = t0 - 1, *c = c0 - 1;
+    for (int i = 1; i <= k1; ++i) { t[i] = x[1]; t[n - k1 + i] = x[m]; }
+    /* fpcurf, s = 0: interior knots at data points (odd k) or midway between them (even k) */
+    int mk1 = m - k1, k3 = k / 2, i = k2, j = k3 + 2;
+    for (int l = 1; l <= mk1; ++l, ++i, ++j) t[i] = (k3 * 2 == k) ? (x[j] + x[j - 1]) * 0.5 : x[j];
+    double *a = calloc((size_t)nk1 * k1, sizeof(double)), *z = calloc((size_t)nk1 + 1, sizeof(double));
+#define A_(r, q) a[(size_t)((r) - 1) * k1 + ((q) - 1)]
+    double h[8];
+    int l = k1;
+    for (int it = 1; it <= m; ++it) {
+        double xi = x[it], yi = y[it];
+        while (!(xi < t[l + 1] || l == nk1)) ++l;
+        fpbspl(t, k, xi, l, h);
+        int jj = l - k1;
+        for (int ii = 1; ii <= k1; ++ii) {
+            ++jj;
+            double piv = h[ii], cs, sn;
+            if (piv == 0.0) continue;
+            fpgivs(piv, &A_(jj, 1), &cs, &sn);
+            fprota(cs, sn, &yi, &z[jj]);
+            if (ii == k1) continue;
+            int i2 = 1;
+            for (int i1 = ii + 1; i1 <= k1; ++i1) {
+                ++i2;
+                fprota(cs, sn, &h[i1], &A_(jj, i2));
+            }
+        }
+    }
+    /* fpback: upper triangular band system of bandwidth k1 */
+    c[nk1] = z[nk1] / A_(nk1, 1);
+    int ib = nk1 - 1;
+    for (int jb = 2; jb <= nk1 && ib >= 1; ++jb) {
+        double store = z[ib];
+        int i1 = (jb <= k1 - 1) ? jb - 1 : k1 - 1, mm = ib;
+        for (int q = 1; q <= i1; ++q) {
+            ++mm;
+            store = store - c[mm] * A_(ib, q + 1);
+        }
+        c[ib] = store / A_(ib, 1);
+        --ib;
+    }
+#undef A_
+    free(a); free(z);
+    return n;
+}
+
+/* splev at one point (Dierckx.jl evaluates one point per call, bc = "extrapolate"): interval from l = k + 1
+ * upward while x >= t(l + 1) and l < n - k - 1 */
+double orc_spline_eval(const double *t0, int n, const double *c0, int k, double arg) {
+    const double *t = t0 - 1, *c = c0 - 1;
+    const int k1 = k + 1, nk1 = n - k1;
+    int l = k1, l1 = l + 1;
+    while (!(arg < t[l1] || l == nk1)) { l = l1; l1 = l + 1; }
+    double h[8];
+    fpbspl(t, k, arg, l, h);
+    double sp = 0.0;
+    int ll = l - k1;
+    for (int j = 1; j <= k1; ++j) { ++ll; sp = sp + c[ll] * h[j]; }
+    return sp;
+}
+
+/* Julia's range start:step:stop for Float64 (base/twiceprecision.jl): when start, step and stop have exact small
+ * rationals (rat, continued fractions bounded by maxintfloat(Float32)) the elements are (start_n + i step_n) / den
+ * to twice precision, i.e. the correctly rounded quotient; otherwise start + i*step taken literally (the product
+ * rounded, then the sum rounded). */
+static void jl_rat(double x, int64_t *num, int64_t *den) {
+    const double mf = 16777216.0;
+    double y = x;
+    int64_t a = 1, d = 1, b = 0, c = 0;
+    while (fabs(y) <= mf) {
+        int64_t f = (int64_t)trunc(y);
+        y -= (double)f;
+        int64_t a2 = f * a + c, b2 = f * b + d;
+        c = a; a = a2;
+        d = b; b = b2;
+        if (!(llabs(a) <= 16777216 && llabs(b) <= 16777216)) { *num = c; *den = d; return; }
+        if ((double)a / (double)b == x) break;
+        y = 1.0 / y;
+    }
+    *num = a; *den = b;
+}
+static int64_t gcd64(int64_t a, int64_t b) { a = llabs(a); b = llabs(b); while (b) { int64_t r = a % b; a = b; b = r; } return a; }
+static int isbetween(double a, double x, double b) { return (a <= x && x <= b) || (b <= x && x <= a); }
+typedef struct { int rational; int64_t start_n, step_n, den, len; double start, step; } jlrange;
+static void jlrange_make(double start, double step, double stop, jlrange *r) {
+    memset(r, 0, sizeof *r);
+    r->start = start; r->step = step;
+    int64_t step_n, step_d;
+    jl_rat(step, &step_n, &step_d);
+    if (step_d != 0 && (double)step_n / (double)step_d == step) {
+        int64_t start_n, start_d, stop_n, stop_d;
+        jl_rat(start, &start_n, &start_d);
+        jl_rat(stop, &stop_n, &stop_d);
+        if (start_d != 0 && stop_d != 0 && (double)start_n / (double)start_d == start &&
+            (double)stop_n / (double)stop_d == stop) {
+            int64_t den = llabs(start_d / gcd64(start_d, step_d) * step_d);
+            const double mi = 9007199254740992.0;
+            if (den != 0 && fabs(start * (double)den) <= mi && fabs(step * (double)den) <= mi && den % start_d == 0 &&
+                den % step_d == 0) {
+                int64_t sn = llrint(start * (double)den), stn = llrint(step * (double)den);
+                /* Int64 arithmetic wraps in Julia; unsigned arithmetic wraps the same way here */
+                int64_t num = (int64_t)((uint64_t)den * (uint64_t)stop_n - (uint64_t)stop_d * (uint64_t)sn +
+                                        (uint64_t)stn * (uint64_t)stop_d);
+                int64_t len = num / (int64_t)((uint64_t)stn * (uint64_t)stop_d);
+                if (len < 0) len = 0;
+                if (isbetween(start, start + (double)(len - 1) * step, stop + step / 2) &&
+                    !isbetween(start, start + (double)len * step, stop)) {
+                    r->rational = 1; r->start_n = sn; r->step_n = stn; r->den = den; r->len = len;
+                    return;
+                }
+            }
+        }
+    }
+    double lf = (stop - start) / step;
+    int64_t len;
+    if (lf < 0) len = 0;
+    else if (lf == 0) len = 1;
+    else {
+        len = llrint(lf) + 1;
+        double stop2 = start + (double)(len - 1) * step;
+        len -= (start < stop && stop < stop2) + (start > stop && stop > stop2);
+    }
+    r->len = len;
+}
+static double jlrange_at(const jlrange *r, int64_t i) { /* 0-based */
+    if (r->rational) return (double)(r->start_n + i * r->step_n) / (double)r->den;
+    return r->start + (double)i * r->step;
+}
+int64_t orc_jlrange(double start, double step, double stop, double *out, int64_t cap) {
+    jlrange r;
+    jlrange_make(start, step, stop, &r);
+    for (int64_t i = 0; i < r.len && i < cap; ++i) out[i] = jlrange_at(&r, i);
+    return r.len;
+}
+
+/* spline_opt_legacy (src/splines.jl:419-431) / spline_root_legacy (:435-446): the interpolating spline of degree
+ * min(3, m - 1) sampled at knots[1]:0.001:knots[end] (knots[1] = x[0], knots[end] = x[m - 1]); the first sample of
+ * least value (or of least |value - v|) wins */
+static double spline_scan(const double *x, const double *y, int m, int root, double v, double *yout) {
+    int k = m - 1 < 3 ? m - 1 : 3;
+    double *t = malloc(sizeof(double) * (size_t)(m + k + 1)), *c = malloc(sizeof(double) * (size_t)m);
+    int n = orc_spline_interp(x, y, m, k, t, c);
+    jlrange r;
+    jlrange_make(t[k], 0.001, t[n - k - 1], &r);
+    if (!(r.len >= 1 && r.len <= ((int64_t)1 << 24) + 1)) { /* non-finite knots only */
+        free(t); free(c);
+        if (yout) *yout = NAN;
+        return NAN;
+    }
+    double bx = jlrange_at(&r, 0), s = orc_spline_eval(t, n, c, k, bx);
+    double by = root ? fabs(s - v) : s;
+    for (int64_t i = 1; i < r.len; ++i) {
+        double xi = jlrange_at(&r, i), si = orc_spline_eval(t, n, c, k, xi);
+        double yi = root ? fabs(si - v) : si;
+        if (yi < by) { bx = xi; by = yi; }
+    }
+    free(t); free(c);
+    if (yout) *yout = by;
+    return bx;
+}
+void orc_spline_opt_legacy(const double *x, const double *y, int m, double out[2]) {
+    out[0] = spline_scan(x, y, m, 0, 0.0, &out[1]);
+}
+double orc_spline_root_legacy(const double *x, const double *y, int m, double value) {
+    return spline_scan(x, y, m, 1, value, NULL);
+}
+
+/* lsqnonneg_chi2! with method :legacy (src/lsqnonneg.jl:504-532) and chi2_search_from_minimum (:595-636):
+ * mu doubles from 1e-3 until res2(mu) >= chi2 * res2_min, then mu is the sampled root of the spline through
+ * (0, res2_min), (mu_i, res2(mu_i)). The reference keeps doubling without bound; here a voxel still short of the
+ * target after ORC_LEGACY_MAX_DOUBLINGS doublings (mu = 1e-3 * 2^24, a sample grid of 2^24 + 1 points) gets NaN. */
+#define ORC_LEGACY_MAX_DOUBLINGS 24
+static int lsq_chi2_legacy(reg_ws *R, double chi2_target, double *x_out, double *mu_out, double *chi2_out) {
+    const double *xu = plain_solve(R);
+    double res2_min = R->plain.rnorm * R->plain.rnorm;
+    int n = R->n;
+    if (res2_min == 0 || R->plain.nsetp == 0) {
+        memcpy(x_out, xu, sizeof(double) * n);
+        *mu_out = 0; *chi2_out = 1;
+        return 0;
+    }
+    cache_reset(R);
+    double mus[ORC_LEGACY_MAX_DOUBLINGS + 2], r2s[ORC_LEGACY_MAX_DOUBLINGS + 2];
+    int np = 0;
+    mus[np] = 0.0; r2s[np] = res2_min; ++np;
+    double munew = 1e-3;
+    for (int it = 0;; ++it) {
+        cache_solve(R, munew);
+        double r2 = cur_resnorm_sq(R);
+        mus[np] = munew; r2s[np] = r2; ++np;
+        if (r2 >= chi2_target * res2_min) break;
+        if (it == ORC_LEGACY_MAX_DOUBLINGS) return -1;
+        munew *= 2.0;
+    }
+    double mu = orc_spline_root_legacy(mus, r2s, np, chi2_target * res2_min);
+    double res2;
+    if (mu == 0) {
+        res2 = res2_min;
+        memcpy(x_out, xu, sizeof(double) * n);
+    } else {
+        memcpy(x_out, cache_solve(R, mu), sizeof(double) * n);
+        res2 = cur_resnorm_sq(R);
+    }
+    *mu_out = mu;
+    *chi2_out = res2 / res2_min;
+    return 0;
+}
+
 #define REG_API(NAME, CALL)                                                                                      \
     void NAME {                                                                                                  \
         reg_ws R;                                                                                                \
@@ -1521,6 +1778,9 @@ REG_API(orc_lsqnonneg_none(const double *A, const double *b, int m, int n, doubl
 REG_API(orc_lsqnonneg_chi2(const double *A, const double *b, int m, int n, double chi2_target, double *x, double *mu,
                            double *chi2),
         lsq_chi2(&R, chi2_target, x, mu, chi2))
+REG_API(orc_lsqnonneg_chi2_legacy(const double *A, const double *b, int m, int n, double chi2_target, double *x,
+                                  double *mu, double *chi2),
+        if (lsq_chi2_legacy(&R, chi2_target, x, mu, chi2)) { *mu = NAN; *chi2 = NAN; for (int j = 0; j < n; ++j) x[j] = NAN; })
 REG_API(orc_lsqnonneg_mdp(const double *A, const double *b, int m, int n, double delta, double *x, double *mu,
                           double *chi2),
         lsq_mdp(&R, delta, x, mu, chi2))
@@ -1578,6 +1838,7 @@ typedef struct {
     void (*loss)(void *ctx, int I, double *u, double *du);
     void *loss_ctx;
     int *trace; /* if set, the grid indices in evaluation order */
+    int legacy; /* CubicSplineSurrogate with spline_opt_legacy (legacy = true) instead of the Hermite surrogate */
 } flip_ws;
 
 static void flip_ws_init(flip_ws *F, int m, int n, int K) {
@@ -1684,8 +1945,19 @@ void orc_sorted_corners(const double *alphas, int lo, int hi, double x, int out[
     out[1] = d1 < d0 ? lo : hi;
 }
 
+/* suggest_point of CubicSplineSurrogate (:494-500): spline_opt_legacy over the evaluated points in grid order */
+static void suggest_point_legacy(const flip_ws *F, double *xo, double *uo) {
+    int np = F->npts;
+    double *ps = malloc(sizeof(double) * (size_t)np), *us = malloc(sizeof(double) * (size_t)np), r[2];
+    for (int i = 1; i <= np; ++i) { ps[i - 1] = F->alphas[F->sidx[i] - 1]; us[i - 1] = F->u[F->sidx[i]]; }
+    orc_spline_opt_legacy(ps, us, np, r);
+    free(ps); free(us);
+    *xo = r[0]; *uo = r[1];
+}
+
 /* suggest_point (:544-566) */
 static void suggest_point(const flip_ws *F, double *xo, double *uo) {
+    if (F->legacy) { suggest_point_legacy(F, xo, uo); return; }
     int I0 = F->sidx[1];
     double plast = F->alphas[I0 - 1], ulast = F->u[I0], dlast = F->du[I0];
     double p = plast, u = ulast;
@@ -1745,6 +2017,21 @@ double orc_surrogate_search(const double *alphas, int K, int mineval,
     return a;
 }
 
+double orc_surrogate_search_legacy(const double *alphas, int K, int mineval,
+                                   void (*loss)(void *ctx, int I, double *u, double *du), void *ctx, int *trace,
+                                   int *nevals) {
+    flip_ws F;
+    flip_ws_init(&F, 1, 1, K);
+    F.alphas = alphas;
+    F.loss = loss;
+    F.loss_ctx = ctx;
+    F.trace = trace;
+    F.legacy = 1;
+    double a = flip_search(&F, NULL, mineval, nevals);
+    flip_ws_free(&F);
+    return a;
+}
+
 double orc_flip_search(const double *As, const double *dAs, const double *alphas, int m, int n, int K, int mineval,
                        const double *b, int *nevals) {
     flip_ws F;
@@ -1796,6 +2083,7 @@ static void *pipe_worker(void *arg) {
     int search = isnan(o->SetFlipAngle) && P->alpha_in == NULL;
     flip_ws_init(&F, m, n, P->K);
     F.As = P->As; F.dAs = P->dAs; F.alphas = P->alphas;
+    F.legacy = o->legacy != 0;
     double *A = malloc(sizeof(double) * (size_t)m * n), *b = malloc(sizeof(double) * m), *x = malloc(sizeof(double) * n);
     double *fit = malloc(sizeof(double) * m), *res = malloc(sizeof(double) * m);
     if (!isnan(o->SetFlipAngle)) memcpy(A, P->As, sizeof(double) * (size_t)m * n);
@@ -1833,7 +2121,10 @@ static void *pipe_worker(void *arg) {
                 if (g_margin_out) g_margin_out[I] = tl_lc_margin;
                 break;
             case ORC_REG_GCV: lsq_gcv(&R, x, &muv, &chi2v); break;
-            case ORC_REG_CHI2: lsq_chi2(&R, o->Chi2Factor, x, &muv, &chi2v); break;
+            case ORC_REG_CHI2:
+                if (!o->legacy) lsq_chi2(&R, o->Chi2Factor, x, &muv, &chi2v);
+                else if (lsq_chi2_legacy(&R, o->Chi2Factor, x, &muv, &chi2v)) continue; /* voxel left NaN */
+                break;
             default: {
                 double sigma = o->NoiseLevel / max_signal;
                 double delta = sqrt((double)m) * sigma;

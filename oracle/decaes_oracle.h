@@ -15,15 +15,22 @@
  *   orc_nnls, _tikh      scipy.optimize.nnls vectors; brute force over all active sets; KKT; Tikhonov gradient
  *   orc_brent_*, bracket sin / sin^2 roots and minima (the reference's test/optimization.jl cases)
  *   orc_hermite_minimize exact minimum of cubics
- *   orc_lcurve_corner    mock (mu, 1/mu) curve: log mu* = 0 within 1e-3 (the reference's test/nnls.jl case)
+ *   orc_surrogate_search the reference's hand-checked bounding boxes; evaluation sequences derived by hand on a
+ *                        quadratic loss (K = 32, 64; minimum left of the grid)
+ *   orc_lcurve_corner    mock (mu, 1/mu) curve: log mu* = 0 within 1e-3 (the reference's test/nnls.jl case); the
+ *                        first golden-section moves by hand; corners recovered across the bracket; the Ctol rule;
+ *                        backtracking on/off identical
+ *   orc_spline_*         scipy's FITPACK (splrep / splev, s = 0) bit for bit; exact on cubics
+ *   orc_jlrange          Julia's documented range lengths and decimal endpoints; correctly rounded k / 1000
+ *   chi2 legacy          the doubling search and the sampled root rebuilt from orc_nnls_tikh and scipy's spline
  *   orc_svdvals          numpy SVD of EPG bases; orc_gcv_dof: trace formula
  *   orc_lsqnonneg_*      chi2 / mdp targets met; gcv optimum is a local minimum; lcurve solution is the
  *                        Tikhonov solution at its mu; none is plain NNLS
  *   orc_t2part, erfinv   closed forms on hand-built distributions; scipy.special.erfinv
  *   orc_t2map            noiseless on-grid voxels recover alpha; threshold and NaN semantics
- * Parity unpinned against the reference itself (no reference-held vectors; Julia cannot run here): the
- * bisection trajectory of orc_flip_search, mu beyond solver tolerance in orc_lsqnonneg_{chi2,mdp,gcv}, the
- * L-curve corner beyond the mock case, and hence the end-to-end maps of orc_t2map.
+ * Parity unpinned against the reference itself (no reference-held vectors; Julia cannot run here): mu beyond
+ * solver tolerance in orc_lsqnonneg_{chi2,mdp,gcv}, LAPACK's singular values beyond SVD equivalence, and hence the
+ * end-to-end maps of orc_t2map against Julia DECAES.
  */
 #ifndef DECAES_ORACLE_H
 #define DECAES_ORACLE_H
@@ -50,7 +57,7 @@ typedef struct {
     int32_t nRefAngles;
     int32_t nRefAnglesMin;
     int32_t Reg;
-    int32_t _pad0;
+    int32_t legacy;
     double Chi2Factor;
     double NoiseLevel;
     double RefConAngle;
@@ -145,9 +152,28 @@ double orc_flip_search(const double *As, const double *dAs, const double *alphas
 double orc_surrogate_search(const double *alphas, int K, int mineval,
                             void (*loss)(void *ctx, int I, double *u, double *du), void *ctx, int *trace,
                             int *nevals);
+/* The same searcher with the legacy surrogate (CubicSplineSurrogate + spline_opt_legacy, src/splines.jl:494-500). */
+double orc_surrogate_search_legacy(const double *alphas, int K, int mineval,
+                                   void (*loss)(void *ctx, int I, double *u, double *du), void *ctx, int *trace,
+                                   int *nevals);
 /* minimal_bounding_box / sorted_corners of the 1-D searcher; seen is indexed 1..K */
 void orc_minimal_bounding_box(const unsigned char *seen, const double *alphas, int K, double x, int *lo, int *hi);
 void orc_sorted_corners(const double *alphas, int lo, int hi, double x, int out[2]);
+
+/* ---- legacy path (legacy = true; src/splines.jl:312-315,414-446; src/lsqnonneg.jl:521-532,595-636) ---- */
+/* Dierckx.Spline1D(x, y; k, s = 0): FITPACK's interpolating spline (knots of fpcurf's s = 0 case, Givens
+ * rotations, back substitution). t receives m + k + 1 knots, c m coefficients; returns the knot count. */
+int orc_spline_interp(const double *x, const double *y, int m, int k, double *t, double *c);
+/* splev at one point (bc = "extrapolate") */
+double orc_spline_eval(const double *t, int n, const double *c, int k, double x);
+/* Julia's Float64 range start:step:stop: returns its length, writes min(length, cap) elements */
+int64_t orc_jlrange(double start, double step, double stop, double *out, int64_t cap);
+/* spline_opt_legacy: out = (x, y) of the least sampled value; spline_root_legacy: the sample nearest value */
+void orc_spline_opt_legacy(const double *x, const double *y, int m, double out[2]);
+double orc_spline_root_legacy(const double *x, const double *y, int m, double value);
+/* lsqnonneg_chi2! with legacy = true; a search still short of the target after 24 doublings gives NaN */
+void orc_lsqnonneg_chi2_legacy(const double *A, const double *b, int m, int n, double chi2_target, double *x,
+                               double *mu, double *chi2);
 
 /* ---- whole pipeline (src/T2mapSEcorr.jl:148-238,512-591) ---- */
 /* image: (nx,ny,nz,nTE) column-major. alpha_in: B1 map (nx,ny,nz) or NULL. Outputs are caller-allocated and

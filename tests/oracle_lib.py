@@ -32,7 +32,7 @@ class T2mapOpts(C.Structure):
         ("nTE", C.c_int32), ("nT2", C.c_int32),
         ("TE", C.c_double), ("T2Range", C.c_double * 2), ("T1", C.c_double), ("Threshold", C.c_double),
         ("MinRefAngle", C.c_double), ("nRefAngles", C.c_int32), ("nRefAnglesMin", C.c_int32),
-        ("Reg", C.c_int32), ("_pad0", C.c_int32),
+        ("Reg", C.c_int32), ("legacy", C.c_int32),
         ("Chi2Factor", C.c_double), ("NoiseLevel", C.c_double), ("RefConAngle", C.c_double),
         ("SetFlipAngle", C.c_double),
     ]
@@ -163,6 +163,8 @@ def lsqnonneg(A, b, reg, param=None):
         L.orc_lsqnonneg_none(*args, _p(x), C.byref(mu), C.byref(chi2))
     elif reg == "chi2":
         L.orc_lsqnonneg_chi2(*args, C.c_double(param), _p(x), C.byref(mu), C.byref(chi2))
+    elif reg == "chi2_legacy":
+        L.orc_lsqnonneg_chi2_legacy(*args, C.c_double(param), _p(x), C.byref(mu), C.byref(chi2))
     elif reg == "mdp":
         L.orc_lsqnonneg_mdp(*args, C.c_double(param), _p(x), C.byref(mu), C.byref(chi2))
     elif reg == "gcv":
@@ -243,9 +245,10 @@ def flip_search(As, dAs, alphas, b, mineval):
 LOSS = C.CFUNCTYPE(None, C.c_void_p, C.c_int, _dp, _dp)
 
 
-def surrogate_search(alphas, mineval, loss):
-    """The flip-angle searcher on loss(alpha) -> (u, du/dalpha) over the grid alphas. Returns (x, trace), trace
-    the 1-based grid indices in evaluation order."""
+def surrogate_search(alphas, mineval, loss, legacy=False):
+    """The flip-angle searcher on loss(alpha) -> (u, du/dalpha) over the grid alphas (with legacy, the cubic spline
+    surrogate sampled on a 0.001 grid; du unused). Returns (x, trace), trace the 1-based grid indices in evaluation
+    order."""
     alphas = np.ascontiguousarray(alphas, dtype=np.float64)
     K = len(alphas)
 
@@ -257,10 +260,58 @@ def surrogate_search(alphas, mineval, loss):
     trace = np.zeros(K, dtype=np.int32)
     nev = C.c_int(0)
     L = lib()
-    L.orc_surrogate_search.restype = C.c_double
-    x = L.orc_surrogate_search(_p(alphas), C.c_int(K), C.c_int(mineval), cb, None, trace.ctypes.data_as(_ip),
-                               C.byref(nev))
+    fn = L.orc_surrogate_search_legacy if legacy else L.orc_surrogate_search
+    fn.restype = C.c_double
+    x = fn(_p(alphas), C.c_int(K), C.c_int(mineval), cb, None, trace.ctypes.data_as(_ip), C.byref(nev))
     return x, [int(i) for i in trace[:nev.value]]
+
+
+def spline_interp(x, y, k):
+    """Dierckx.Spline1D(x, y; k, s = 0): (knots, coefficients)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    m = len(x)
+    t = np.empty(m + k + 1)
+    c = np.empty(m)
+    n = lib().orc_spline_interp(_p(x), _p(y), C.c_int(m), C.c_int(k), _p(t), _p(c))
+    assert n == m + k + 1, n
+    return t, c
+
+
+def spline_eval(t, c, k, x):
+    L = lib()
+    L.orc_spline_eval.restype = C.c_double
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    return np.array([L.orc_spline_eval(_p(t), C.c_int(len(t)), _p(c), C.c_int(k), C.c_double(float(v)))
+                     for v in np.atleast_1d(x)])
+
+
+def jlrange(start, step, stop, cap=None):
+    """Julia's Float64 range start:step:stop as an array (its first cap elements when cap is given)."""
+    L = lib()
+    L.orc_jlrange.restype = C.c_int64
+    n = L.orc_jlrange(C.c_double(start), C.c_double(step), C.c_double(stop), None, C.c_int64(0))
+    cap = n if cap is None else min(n, cap)
+    out = np.empty(max(cap, 1))
+    L.orc_jlrange(C.c_double(start), C.c_double(step), C.c_double(stop), _p(out), C.c_int64(cap))
+    return out[:cap], n
+
+
+def spline_opt_legacy(x, y):
+    out = (C.c_double * 2)()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    lib().orc_spline_opt_legacy(_p(x), _p(y), C.c_int(len(x)), out)
+    return out[0], out[1]
+
+
+def spline_root_legacy(x, y, value):
+    L = lib()
+    L.orc_spline_root_legacy.restype = C.c_double
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    return L.orc_spline_root_legacy(_p(x), _p(y), C.c_int(len(x)), C.c_double(value))
 
 
 def minimal_bounding_box(seen_indices, alphas, x):
@@ -282,7 +333,8 @@ def sorted_corners(alphas, lo, hi, x):
 
 
 def make_t2map_opts(shape, nTE, TE, nT2, T2Range, Reg, T1=1.0, Threshold=0.0, MinRefAngle=50.0, nRefAngles=64,
-                    nRefAnglesMin=None, Chi2Factor=None, NoiseLevel=None, RefConAngle=180.0, SetFlipAngle=None):
+                    nRefAnglesMin=None, Chi2Factor=None, NoiseLevel=None, RefConAngle=180.0, SetFlipAngle=None,
+                    legacy=False):
     o = T2mapOpts()
     o.nx, o.ny, o.nz = shape
     o.nTE, o.nT2, o.TE = nTE, nT2, TE
@@ -295,6 +347,7 @@ def make_t2map_opts(shape, nTE, TE, nT2, T2Range, Reg, T1=1.0, Threshold=0.0, Mi
     o.NoiseLevel = math.nan if NoiseLevel is None else NoiseLevel
     o.RefConAngle = RefConAngle
     o.SetFlipAngle = math.nan if SetFlipAngle is None else SetFlipAngle
+    o.legacy = int(bool(legacy))
     return o
 
 
