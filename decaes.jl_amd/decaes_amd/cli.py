@@ -6,15 +6,20 @@
 
 For each input file: load the 4D image (MAT or NIfTI), multiply by the mask file if given, run T2mapSEcorr (with the
 B1 map if given) and/or T2partSEcorr, and write <name>.t2dist.mat, <name>.t2maps.mat, <name>.t2parts.mat and
-<name>.log into the output folder, copying each @settings file there as <name>.<settings file name>. A failure on one
-file is logged and the next file is processed. --bet (FSL BET) and PAR/REC inputs are not available in this build.
+<name>.log into the output folder (MAT v7.3, as the reference's MAT.jl writes them), copying each @settings file there
+as <name>.<settings file name>. A failure on one file is logged and the next file is processed. --bet runs the FSL BET
+executable given by --betpath, as the reference does; PAR/REC inputs are not available in this build.
 """
 from __future__ import annotations
 
 import argparse
 import logging
+import os
+import re
 import shutil
+import subprocess
 import sys
+import tempfile
 import time
 import warnings
 from pathlib import Path
@@ -145,6 +150,30 @@ def get_file_infos(o: dict, settings_files) -> list:
                  settings=list(settings_files)) for i, f in enumerate(files)]
 
 
+def clean_bet_args(betargs) -> list:
+    """clean_bet_args (src/main.jl:692-700): split on commas and spaces; "-m" (make a binary mask) is always there."""
+    if not isinstance(betargs, str):
+        return list(betargs)
+    args = [a for a in re.split(r"[\s,]+", betargs) if a]
+    if "-m" not in args:
+        args.insert(0, "-m")
+    return args
+
+
+def make_bet_mask(image3: np.ndarray, betpath: str, betargs: list) -> np.ndarray:
+    """make_bet_mask (src/main.jl:664-681): run FSL's BET executable on the image (the first echo) written as NIfTI
+    in a temporary folder and load the one mask file it writes (BET appends "_mask" and ".nii.gz" to the output
+    name)."""
+    with tempfile.TemporaryDirectory() as tmp:
+        base = Path(tempfile.mktemp(dir=tmp)).name
+        src, dst = Path(tmp) / (base + ".nii"), Path(tmp) / (base + ".bet")
+        fileio.write_nifti(src, image3)
+        subprocess.run([betpath, str(src), str(dst), *betargs], check=True)
+        found = sorted(f for f in os.listdir(tmp) if f.startswith(base + ".bet_mask"))
+        assert len(found) == 1, found
+        return fileio.load_image(Path(tmp) / found[0], 3)
+
+
 def run_file(info: dict, o: dict, log: logging.Logger) -> None:
     t0 = time.time()
     log.info(f"Loading input file: {info['inputfile']}")
@@ -156,7 +185,12 @@ def run_file(info: dict, o: dict, log: logging.Logger) -> None:
         except Exception as e:  # the reference warns and continues unmasked
             log.warning(f"Error while loading mask file: {info['maskfile']}: {e}")
     elif o["bet"]:
-        log.warning("Error while making mask using BET: FSL BET is not available in this build; continuing unmasked")
+        args = clean_bet_args(o["betargs"])
+        log.info(f"Making and applying BET mask with args: {' '.join(args)}")
+        try:
+            image *= make_bet_mask(image[..., 0], o["betpath"], args)[..., None]
+        except Exception as e:  # the reference warns and continues unmasked
+            log.warning(f"Error while making mask using BET: {e}")
     base = Path(info["outputfolder"]) / info["choppedinputfile"]
     if o["T2map"]:
         opts = T2mapOptions.from_image(image, Silent=o["quiet"], **_kwargs(o, _T2MAP_FIELDS))

@@ -1,8 +1,8 @@
 """Image and result files for the command line layer (src/main.jl:577-604 load_image, :377-411 MAT output).
 
-* MAT: read with scipy.io (MAT v4/v5/v7 files; v7.3 files are HDF5 and need h5py, which is not part of this build).
-  Results are written as MAT v5 with the reference's keys and array shapes (MATLAB, scipy and MAT.jl all read v5;
-  the reference's MAT.jl writes v7.3).
+* MAT: results are written as MAT v7.3 (HDF5, through the HDF5 C library; decaes_amd.mat73) with the reference's
+  keys and array shapes, as MAT.jl's matwrite writes them; where libhdf5 is missing they are written as MAT v5 with
+  a warning. Inputs: v7.3 files through libhdf5, v4/v5/v7 files through scipy.io.
 * NIfTI-1 (.nii, .nii.gz): a reader and writer of the single-file format, with scl_slope / scl_inter applied as the
   reference does (slope 0 means unscaled).
 * PAR/REC: not supported by this build.
@@ -109,12 +109,7 @@ def load_image(filename, ndims: int = 4) -> np.ndarray:
     Fortran order."""
     ext = maybe_get_suffix(filename)
     if ext == ".mat":
-        import scipy.io
-
-        try:
-            data = scipy.io.loadmat(filename)
-        except NotImplementedError as e:
-            raise NotImplementedError(f"{filename}: MAT v7.3 (HDF5) files are not supported by this build ({e})")
+        data = read_mat(filename)
         keys = sorted(k for k, v in data.items() if not k.startswith("__") and isinstance(v, np.ndarray)
                       and v.ndim == ndims)
         if not keys:
@@ -133,9 +128,27 @@ def load_image(filename, ndims: int = 4) -> np.ndarray:
     return np.asfortranarray(arr, dtype=np.float64)
 
 
-def save_mat(filename, arrays: dict) -> None:
-    """MAT v5 file with one variable per key (arrays keep their shapes; scalars become 1x1)."""
+def read_mat(filename) -> dict:
+    """The variables of a MAT file (v7.3 through libhdf5, older versions through scipy.io)."""
+    from . import mat73
+
+    if mat73.is_mat73(filename):
+        return mat73.read(filename)
     import scipy.io
 
+    return scipy.io.loadmat(filename)
+
+
+def save_mat(filename, arrays: dict) -> None:
+    """MAT file with one float64 variable per key, arrays keeping their shapes: v7.3 as MAT.matwrite writes it
+    (src/main.jl:382,391,411), or v5 when the HDF5 library is missing."""
+    from . import mat73
+
+    if mat73.available():
+        mat73.write(filename, arrays)
+        return
+    import scipy.io
+
+    warnings.warn("libhdf5 not found: writing MAT v5 instead of v7.3")
     scipy.io.savemat(filename, {k: (np.asfortranarray(v) if isinstance(v, np.ndarray) else v)
                                 for k, v in arrays.items()}, oned_as="column")

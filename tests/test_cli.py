@@ -2,6 +2,7 @@
 import gzip
 import struct
 import warnings
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -110,3 +111,26 @@ def test_file_infos(tmp_path):
         cli.get_file_infos(cli.parse_cli(["a/x.mat", "b/y.mat", "c/z.mat", "--T2part", "-o", "p", "q"]), [])
     with pytest.raises(AssertionError, match="SetFlipAngle"):
         cli.get_file_infos(cli.parse_cli(["a/x.mat", "--T2map", "--B1map", "b.mat", "--SetFlipAngle", "170"]), [])
+
+
+def test_bet_args_and_mask(tmp_path):
+    """clean_bet_args (src/main.jl:692-700) and make_bet_mask (:664-681) with a stand-in BET executable that writes
+    <out>_mask.nii.gz as FSL's BET does (FSL is not installed here): the first echo goes in as NIfTI, the mask comes
+    back and multiplies the image."""
+    import sys
+    from decaes_amd import cli
+
+    assert cli.clean_bet_args("-m -n -f 0.25 -R") == ["-m", "-n", "-f", "0.25", "-R"]
+    assert cli.clean_bet_args("-n,-f,0.5") == ["-m", "-n", "-f", "0.5"]
+    fake = tmp_path / "fakebet"
+    fake.write_text(f"#!{sys.executable}\n"
+                    "import sys\n"
+                    f"sys.path.insert(0, {str(Path(cli.__file__).parents[1])!r})\n"
+                    "from decaes_amd import fileio\n"
+                    "img = fileio.load_image(sys.argv[1], 3)\n"
+                    "assert '-m' in sys.argv[3:]\n"
+                    "fileio.write_nifti(sys.argv[2] + '_mask.nii.gz', (img > 0.5).astype(float))\n")
+    fake.chmod(0o755)
+    img = np.random.default_rng(0).uniform(size=(5, 4, 3))
+    mask = cli.make_bet_mask(img, str(fake), cli.clean_bet_args("-n"))
+    np.testing.assert_array_equal(mask, (img > 0.5).astype(float))

@@ -15,14 +15,16 @@ def _image(dmi, seed=3):
     return img
 
 
-@pytest.mark.parametrize("fmt,settings", [(".mat", False), (".nii.gz", True)])
+@pytest.mark.parametrize("fmt,settings", [(".mat", False), ("v7.3 .mat", False), (".nii.gz", True)])
 def test_cli_matches_api(dmi, tmp_path, fmt, settings):
-    from decaes_amd import cli, fileio
+    from decaes_amd import cli, fileio, mat73
     print(f"[cli] {fmt} settings={settings}", flush=True)
     img = _image(dmi)
-    inp = tmp_path / ("scan" + fmt)
+    inp = tmp_path / ("scan" + fmt.split()[-1])
     if fmt == ".mat":
         scipy.io.savemat(inp, {"img": img})
+    elif fmt == "v7.3 .mat":
+        mat73.write(inp, {"img": img})
     else:
         fileio.write_nifti(inp, img)
     out = tmp_path / "out"
@@ -34,17 +36,19 @@ def test_cli_matches_api(dmi, tmp_path, fmt, settings):
     assert cli.main(args) == 0
     for ext in (".t2dist.mat", ".t2maps.mat", ".t2parts.mat", ".log"):
         assert (out / ("scan" + ext)).is_file(), ext
+    for ext in (".t2dist.mat", ".t2maps.mat", ".t2parts.mat"):
+        assert mat73.is_mat73(out / ("scan" + ext)), ext  # MAT v7.3, as MAT.matwrite writes
     assert (out / "scan.settings.txt").is_file() == settings
     opts = dmi.T2mapOptions.from_image(img, TE=0.01, nT2=40, T2Range=(0.01, 2.0), Reg="chi2", Chi2Factor=1.02,
                                        nRefAngles=32, SaveRegParam=True, Silent=True)
     maps, dist = dmi.T2mapSEcorr(img, opts)
     parts = dmi.T2partSEcorr(dist, dmi.T2partOptions.from_t2map(opts, SPWin=(0.01, 0.04), MPWin=(0.04, 2.0)))
-    np.testing.assert_array_equal(scipy.io.loadmat(out / "scan.t2dist.mat")["dist"], dist)
-    m = scipy.io.loadmat(out / "scan.t2maps.mat")
+    np.testing.assert_array_equal(fileio.read_mat(out / "scan.t2dist.mat")["dist"], dist)
+    m = fileio.read_mat(out / "scan.t2maps.mat")
     assert set(k for k in m if not k.startswith("__")) == set(maps)
     for k in ("gdn", "ggm", "gva", "fnr", "snr", "alpha", "mu", "chi2factor", "decaybasisset"):
         np.testing.assert_array_equal(m[k], maps[k])
-    p = scipy.io.loadmat(out / "scan.t2parts.mat")
+    p = fileio.read_mat(out / "scan.t2parts.mat")
     for k in ("sfr", "sgm", "mfr", "mgm"):
         np.testing.assert_array_equal(p[k], parts[k])
 
@@ -63,17 +67,17 @@ def test_cli_t2part_only_mask_b1(dmi, tmp_path):
     assert cli.main([str(tmp_path / "scan.mat"), "--T2map", "-o", str(out), "-q", "--Reg", "none", "-m",
                      str(tmp_path / "mask.nii"), "--B1map", str(tmp_path / "b1.nii")] + ARGS) == 0
     assert not (out / "scan.t2parts.mat").exists()
-    m = scipy.io.loadmat(out / "scan.t2maps.mat")
+    m = fileio.read_mat(out / "scan.t2maps.mat")
     assert np.all(np.isnan(m["gdn"][0])) and np.all(m["alpha"] == 165.0)
     opts = dmi.T2mapOptions.from_image(img * mask[..., None], TE=0.01, nT2=40, T2Range=(0.01, 2.0), Reg="none",
                                        nRefAngles=32, SaveRegParam=True, Silent=True)
     maps, dist = dmi.T2mapSEcorr(img * mask[..., None], opts, alpha=b1)
-    np.testing.assert_array_equal(scipy.io.loadmat(out / "scan.t2dist.mat")["dist"], dist)
+    np.testing.assert_array_equal(fileio.read_mat(out / "scan.t2dist.mat")["dist"], dist)
     # T2part only: the input is the distribution; nT2 is inferred from it
     scipy.io.savemat(tmp_path / "d.mat", {"dist": dist})
     assert cli.main([str(tmp_path / "d.mat"), "--T2part", "-o", str(out), "-q", "--T2Range", "0.01", "2.0",
                      "--SPWin", "0.01", "0.04", "--MPWin", "0.04", "2.0", "--Sigmoid", "0.005"]) == 0
-    p = scipy.io.loadmat(out / "d.t2parts.mat")
+    p = fileio.read_mat(out / "d.t2parts.mat")
     ref = dmi.T2partSEcorr(dist, T2Range=(0.01, 2.0), SPWin=(0.01, 0.04), MPWin=(0.04, 2.0), Sigmoid=0.005)
     for k in ("sfr", "sgm", "mfr", "mgm"):
         np.testing.assert_array_equal(p[k], ref[k])
