@@ -9,7 +9,7 @@ FP64 path timed over the whole volume (plain_fp64), the metric's accuracy clause
 against the plain path, and the plain path's own change under a 1-ulp input change), the p50/p99 latency of a served
 stream below capacity, the roofline object and the CPU baseline.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1] [--precision mixed|fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c3chi2|c2|c1] [--precision mixed|fp64]
                     [--slabs N] [--no-cpu-baseline] [--no-served] [--no-plain] [--no-ulp]
     python bench.py --copy-probe BYTES    # plain device copy of known size, the HBM-counter calibration run
     python bench.py --serve [--qps 2,8,32] [--requests 16] [--max-slices 8]   # served stream: p50/p99 latency
@@ -38,6 +38,10 @@ WORKLOADS = {
     # 240x240x113, nTE 56, TE 7.9 ms, nT2 60 over 8 ms-2 s, lcurve, 64 angles, 4 compartments
     "c3": dict(shape=(240, 240, 113), nTE=56, TE=7.9e-3, nT2=60, T2Range=(8e-3, 2.0), Reg="lcurve", nRefAngles=64,
                ncomp=4, mask=0.6, Chi2Factor=None, slabs=8),
+    # the same volume with chi2 1.02, the regularization of the MATLAB toolbox that the reference's published
+    # whole-volume timings compare against
+    "c3chi2": dict(shape=(240, 240, 113), nTE=56, TE=7.9e-3, nT2=60, T2Range=(8e-3, 2.0), Reg="chi2", nRefAngles=64,
+                   ncomp=4, mask=0.6, Chi2Factor=1.02, slabs=8),
 }
 SPWIN = (None, 40e-3)  # short-peak window upper edge (lower edge = T2Range[0])
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
