@@ -294,6 +294,53 @@ def test_full_volume_sampled(dmi, oracle):
     assert (~stable).sum() <= 0.02 * stable.size, s
 
 
+def test_c3_volume_plain_and_mixed(dmi, oracle):
+    """The 240x240x113, 56-echo, nT2 60, L-curve, 64-angle volume of the headline (3.9M voxels above threshold),
+    plain FP64 and mixed precision, through the host-array entry point: invariants on every voxel, the plain path
+    against the oracle on a seeded sample of 1500 in-mask voxels (per-voxel bar, rounding ties bounded), and the
+    mixed path against the plain one on every voxel: MWF within 1e-3 on at least 99.99% of voxels and never more
+    than 0.05, alpha within 0.1 degree."""
+    shape, m, TE, n, T2R = (240, 240, 113), 56, 7.9e-3, 60, (8e-3, 2.0)
+    print(f"[parity] C3 volume {shape} nTE={m} lcurve", flush=True)
+    img, nin = dmi.synth_mse(shape, m, TE, ncomp=4, mask_fraction=0.6, seed=2024)
+    kw = dict(TE=TE, nT2=n, T2Range=T2R, Reg="lcurve", nRefAngles=64, SaveRegParam=True, Silent=True)
+    maps, dist = dmi.T2mapSEcorr(img, dmi.T2mapOptions.from_image(img, **kw))
+    inmask = img[..., 0] > 0
+    assert int(inmask.sum()) == nin
+    assert np.array_equal(~np.isnan(maps["gdn"]), inmask)
+    g = maps["gdn"][inmask]
+    assert np.all(np.isfinite(g)) and np.all(g > 0)
+    assert np.max(np.abs(dist[inmask].sum(axis=1) - g) / g) <= 1e-12
+    # the sampled voxels packed into a (N, 1, 1) image for the oracle (voxels are independent of their position)
+    rng = np.random.default_rng(4321)
+    lin = np.flatnonzero(inmask.ravel(order="F"))
+    vox = np.sort(rng.choice(lin, size=1500, replace=False))
+    idx = np.unravel_index(vox, shape, order="F")
+    N = len(vox)
+    pimg = np.asfortranarray(img[idx][:, None, None, :])
+    pmaps = {k: np.asfortranarray(v[idx][:, None, None]) for k, v in maps.items() if v.shape == shape}
+    pdist = np.asfortranarray(dist[idx][:, None, None, :])
+    oo = oracle.make_t2map_opts((N, 1, 1), m, TE, n, T2R, "lcurve", nRefAngles=64)
+    omaps, odist, sel, spread = PU.oracle_ensemble(oracle, pimg, oo, T2R)
+    assert sel.all()
+    s, stable = PU.check("c3_volume_sampled", oracle, T2R, sel, spread, pmaps, pdist, omaps, odist)
+    spw = dict(SPWin=(T2R[0], 40e-3), MPWin=(40e-3, T2R[1]), Silent=True)
+    sfr = dmi.T2partSEcorr(dist, dmi.T2partOptions.from_dist(dist, T2Range=T2R, **spw))["sfr"][inmask]
+    alpha = maps["alpha"][inmask]
+    del maps, dist
+    mmaps, mdist = dmi.T2mapSEcorr(img, dmi.T2mapOptions.from_image(img, precision="mixed", **kw))
+    msfr = dmi.T2partSEcorr(mdist, dmi.T2partOptions.from_dist(mdist, T2Range=T2R, **spw))["sfr"][inmask]
+    err = np.abs(msfr - sfr)
+    da = np.abs(mmaps["alpha"][inmask] - alpha)
+    big = da > 0.1
+    print(f"[parity] mixed vs plain on {err.size} voxels: MWF max {err.max():.3e}, > 1e-3: {(err > 1e-3).sum()}; "
+          f"alpha max {da.max():.3e} deg, > 0.1 deg: {big.sum()}, > 1e-3 deg: {(da > 1e-3).sum()}, "
+          f"MWF of those > 0.1 deg: max {err[big].max(initial=0):.3e}, p50 {np.median(err[big]) if big.any() else 0:.3e}; "
+          f"alpha p99.99 {np.quantile(da, 0.9999):.3e}", flush=True)
+    assert (err > 1e-3).mean() <= 1e-4 and err.max() <= 0.05
+    assert (da > 0.1).mean() <= 1e-4
+
+
 @pytest.mark.parametrize("name,shape,m,n,K,reg,extra", [
     ("c3_lcurve", (10, 10, 2), 56, 60, 64, "lcurve", {}),
     ("c1_chi2", (12, 12, 2), 32, 40, 32, "chi2", dict(Chi2Factor=1.02)),
