@@ -127,6 +127,38 @@ def copy_probe(nbytes: int):
     print(json.dumps({"copy_bytes": a.numel() * 4, "GBps_read_plus_write": 2 * a.numel() * 4 / dt / 1e9}), flush=True)
 
 
+def alone_rate(items, tag="alone") -> dict:
+    """The same requests one at a time through the device-array entry points (image and outputs resident in HBM):
+    total voxels over the summed device time of T2mapSEcorr + T2partSEcorr, and over the summed kernel time.
+    items: iterable of (image, T2mapOptions, T2partOptions)."""
+    import numpy as np
+    import torch
+
+    import decaes_amd as dmi
+    from decaes_amd.device import alloc_t2map_outputs, t2map_device, t2part_device
+
+    ctx = dmi.context(0)
+    t_k, t_all, nv = 0.0, 0.0, 0
+    for img, o, po in items:
+        d_img = torch.from_numpy(np.ascontiguousarray(img.transpose(3, 2, 1, 0))).to("cuda:0")
+        out = alloc_t2map_outputs(o, "cuda:0")
+        X, Y, Z = img.shape[:3]
+        pout = {k: torch.full((Z, Y, X), float("nan"), dtype=torch.float64, device="cuda:0")
+                for k in ("sfr", "sgm", "mfr", "mgm")}
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        stt = t2map_device(d_img, o, out, ctx=ctx)
+        t2part_device(out["dist"], po, pout, ctx=ctx)
+        torch.cuda.synchronize()
+        t_all += time.perf_counter() - ta
+        t_k += stt["kernel_ms"] * 1e-3
+        nv += stt["voxels"]
+        log(f"[{tag}] {tuple(img.shape[:3])} nTE={o.nTE} {o.Reg}: {stt['voxels']} voxels, "
+            f"kernel {stt['kernel_ms']:.0f} ms")
+        del d_img, out, pout
+    return {"voxels": nv, "kernel_voxels_per_s": nv / t_k, "device_voxels_per_s": nv / t_all}
+
+
 def serve_levels(args) -> list:
     """Served stream through the request loop: Poisson arrivals at each QPS of requests of 1..max_slices slices of
     240x240 (nTE in {32, 48, 56}, Reg in {chi2, gcv, lcurve}, T2part on each); p50/p99 latency and voxels/s per
@@ -192,6 +224,10 @@ def serve_levels(args) -> list:
         levels.append({"qps": qps, "requests": len(reqs), "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
                        "voxels": st["voxels"], "voxels_per_s_busy": st["voxels"] / (st["busy_ms"] * 1e-3),
                        "voxels_per_s_wall": st["voxels"] / wall, "failed": st["failed"]})
+        if getattr(args, "compare_alone", False):
+            a = alone_rate(reqs)
+            a["served_busy_over_alone"] = levels[-1]["voxels_per_s_busy"] / a["device_voxels_per_s"]
+            levels[-1]["alone"] = a
         log(f"[serve] qps {qps}: p50 {st['p50_ms']:.0f} ms p99 {st['p99_ms']:.0f} ms, {st['voxels']} voxels, "
             f"{levels[-1]['voxels_per_s_busy']:.0f} voxels/s busy")
     return levels
@@ -247,36 +283,15 @@ def batch(args):
         st = srv.stats()
     alone = None
     if args.compare_alone:
-        # the same volumes one at a time, device-resident (image and outputs in HBM): the single-volume rate of the
-        # same mix is the total voxel count over the summed kernel times
-        import torch
-
-        from decaes_amd.device import alloc_t2map_outputs, t2map_device, t2part_device
-
-        ctx = dmi.context(0)
-        t_k, t_all, nv = 0.0, 0.0, 0
-        for sp in specs:
-            img, nin = dmi.synth_mse(sp["shape"], sp["nTE"], sp["TE"], ncomp=3, mask_fraction=sp["mask"], seed=sp["seed"])
-            o = dmi.T2mapOptions.from_image(img, TE=sp["TE"], nT2=40, T2Range=(10e-3, 2.0), Reg=sp["Reg"],
-                                            Chi2Factor=1.02, Silent=True)
-            po = dmi.T2partOptions.from_t2map(o, SPWin=(10e-3, 40e-3), MPWin=(40e-3, 2.0))
-            d_img = torch.from_numpy(np.ascontiguousarray(img.transpose(3, 2, 1, 0))).to("cuda:0")
-            out = alloc_t2map_outputs(o, "cuda:0")
-            X, Y, Z = sp["shape"]
-            pout = {k: torch.full((Z, Y, X), float("nan"), dtype=torch.float64, device="cuda:0")
-                    for k in ("sfr", "sgm", "mfr", "mgm")}
-            torch.cuda.synchronize()
-            ta = time.perf_counter()
-            stt = t2map_device(d_img, o, out, ctx=ctx)
-            t2part_device(out["dist"], po, pout, ctx=ctx)
-            torch.cuda.synchronize()
-            t_all += time.perf_counter() - ta
-            t_k += stt["kernel_ms"] * 1e-3
-            nv += stt["voxels"]
-            log(f"[alone] {sp['shape']} nTE={sp['nTE']} {sp['Reg']}: {stt['voxels']} voxels, "
-                f"kernel {stt['kernel_ms']:.0f} ms")
-        alone = {"voxels": nv, "kernel_voxels_per_s": nv / t_k, "device_voxels_per_s": nv / t_all,
-                 "batched_busy_over_alone": (voxels / (st["busy_ms"] * 1e-3)) / (nv / t_all)}
+        def items():
+            for sp in specs:
+                img, _ = dmi.synth_mse(sp["shape"], sp["nTE"], sp["TE"], ncomp=3, mask_fraction=sp["mask"],
+                                       seed=sp["seed"])
+                o = dmi.T2mapOptions.from_image(img, TE=sp["TE"], nT2=40, T2Range=(10e-3, 2.0), Reg=sp["Reg"],
+                                                Chi2Factor=1.02, Silent=True)
+                yield img, o, dmi.T2partOptions.from_t2map(o, SPWin=(10e-3, 40e-3), MPWin=(40e-3, 2.0))
+        alone = alone_rate(items())
+        alone["batched_busy_over_alone"] = (voxels / (st["busy_ms"] * 1e-3)) / alone["device_voxels_per_s"]
     print(json.dumps({"metric": "voxels/s (T2map+T2part) batched uneven workload", "value": voxels / wall,
                       "alone": alone,
                       "unit": "voxels/s", "volumes": len(specs), "voxels": voxels, "wall_s": wall,
@@ -310,7 +325,8 @@ def main():
     ap.add_argument("--batch", action="store_true")
     ap.add_argument("--volumes", type=int, default=64)
     ap.add_argument("--window", type=int, default=6)
-    ap.add_argument("--compare-alone", action="store_true", help="--batch: also run each volume alone on the device")
+    ap.add_argument("--compare-alone", action="store_true",
+                    help="--batch / --serve: also run each volume or request alone on the device")
     args = ap.parse_args()
     # PyTorch's HIP runtime is loaded before the library, which then binds to it (one runtime per process)
     import torch  # noqa: F401

@@ -54,6 +54,9 @@ struct Request {
     int stream = -1;
     St st = St::Queued;
     int64_t nvox = -1, launched = 0, finished = 0;
+    bool tune = false;           // measure the launch schedules on this request's first voxels (none measured yet)
+    int per_cu = 0;              // workgroups per CU of the request's chunks
+    int64_t trial = 0;           // voxels run as schedule trials
     unsigned *h_nvox = nullptr;  // pinned
     hipEvent_t ev = nullptr;     // voxel selection done, later: request finished on its stream
     int status = DMI_OK;
@@ -257,13 +260,21 @@ int bind_and_select(dmi_server *S, StreamState &st, Request *r) {
         DMI_HIP(e, dmi::t2map_occupancy(reg, r->o.legacy, &per_cu, lds));
         grid = std::max(grid, per_cu * S->num_cu);
     }
-    // the request's launch schedule (its explicit choice, or the one measured for these sizes, or all-LDS)
+    // the request's launch schedule (its explicit choice, or the one measured for these sizes, or all-LDS until the
+    // first request of these sizes has measured the candidates, see tune_schedule)
     dmi_host::Schedule sched;
     rc = dmi_host::t2map_schedule(e, &r->o, sched);
     if (rc) return rc;
-    const int gld = sched.rl >= m + n ? 0 : sched.gld;
-    const size_t slots = std::max<size_t>((size_t)dmi::t2map_slot_doubles(m, n, 0) * std::max(grid, 1),
-                                          (size_t)dmi::t2map_slot_doubles(m, n, gld) * std::max(sched.per_cu * S->num_cu, 1));
+    std::vector<dmi_host::Schedule> cands;
+    rc = dmi_host::t2map_schedules(e, &r->o, cands);
+    if (rc) return rc;
+    dmi_host::Schedule measured;
+    r->tune = r->o.schedule == DMI_SCHED_AUTO && cands.size() > 1 && !dmi_host::schedule_lookup(&r->o, measured);
+    size_t slots = (size_t)dmi::t2map_slot_doubles(m, n, 0) * std::max(grid, 1);
+    for (const auto &c : cands) {  // room for every candidate, so a trial launch never grows the scratch either
+        const int gld = c.rl >= m + n ? 0 : c.gld;
+        slots = std::max(slots, (size_t)dmi::t2map_slot_doubles(m, n, gld) * std::max(c.per_cu * S->num_cu, 1));
+    }
     DMI_HIP(e, st.lb.scratch.ensure(sizeof(double) * slots));
     DMI_HIP(e, st.lb.counter.ensure(sizeof(unsigned) * 4));
     DMI_HIP(e, st.lb.counters.ensure(sizeof(unsigned long long) * 4));
@@ -276,6 +287,57 @@ int bind_and_select(dmi_server *S, StreamState &st, Request *r) {
                     r->d_out[6], r->d_out[7], r->d_out[8], r->d_out[9], r->d_out[10], r->d_out[11]};
     r->params = dmi_host::t2map_params(&r->o, st.grids.ptrs(), r->d_image, r->d_alpha, &d);
     dmi_host::apply_schedule(r->params, sched);
+    r->per_cu = sched.per_cu;
+    return DMI_OK;
+}
+
+// The autotuner of the one-call path (dmi_api.cpp), for the request loop: the first request of sizes with no
+// measured schedule runs consecutive slices of its voxel list with each candidate schedule (8 voxels per resident
+// workgroup each), timed with events on its stream, and the fastest is kept for the process; every schedule gives
+// the same bits, so the trial voxels are simply done. The scheduler thread waits for the trials (once per set of
+// sizes); chunks already queued on other streams keep running meanwhile.
+int tune_schedule(dmi_server *S, StreamState &st, Request *r) {
+    std::string &e = r->err;
+    std::vector<dmi_host::Schedule> cands;
+    int rc = dmi_host::t2map_schedules(e, &r->o, cands);
+    if (rc) return rc;
+    int64_t need = 0;
+    for (const auto &c : cands) need += 8ll * S->num_cu * c.per_cu;
+    if (r->nvox < 2 * need) return DMI_OK;  // too small to measure: all-LDS for this request
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DMI_HIP(e, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return dmi_host::errorf(e, DMI_EHIP, "hipEventCreate failed");
+    }
+    dmi_host::Schedule best = cands.front();
+    double best_rate = -1.0;
+    int64_t done = 0;
+    for (const auto &c : cands) {
+        const int64_t cnt = 8ll * S->num_cu * c.per_cu;
+        dmi::T2mapParams ps = r->params;
+        dmi_host::apply_schedule(ps, c);
+        float ms = 0.0f;
+        if (hipEventRecord(e0, st.s) != hipSuccess ||
+            (rc = dmi_host::t2map_launch(e, ps, r->d_vox + done, cnt, st.lb, S->num_cu, st.s)) != DMI_OK ||
+            hipEventRecord(e1, st.s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return rc ? rc : dmi_host::errorf(e, DMI_EHIP, "schedule trial failed");
+        }
+        const double rate = cnt / std::max(ms, 1e-3f);
+        if (rate > best_rate) { best_rate = rate; best = c; }
+        done += cnt;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dmi_host::schedule_store(&r->o, best);
+    dmi_host::apply_schedule(r->params, best);
+    r->per_cu = best.per_cu;
+    r->trial = done;
+    r->launched += done;
+    r->finished += done;
     return DMI_OK;
 }
 
@@ -412,6 +474,26 @@ void sched_loop(dmi_server *S) {
                     }
                 }
                 if (!r) break;
+                if (r->tune) {
+                    r->tune = false;
+                    int rc = tune_schedule(S, st, r);
+                    if (rc) {
+                        std::lock_guard<std::mutex> lk(S->mu);
+                        fail_request(r, rc, r->err);
+                    }
+                    progress = true;
+                    if (rc || r->launched >= r->nvox) {
+                        // a failed trial fails the request; either way nothing more is launched for it here
+                        r->finished += r->nvox - r->launched;
+                        r->launched = r->nvox;
+                        if (r->finished == r->nvox) {
+                            (void)launch_finish(st, r);
+                            std::lock_guard<std::mutex> lk(S->mu);
+                            r->st = St::Finishing;
+                        }
+                        continue;
+                    }
+                }
                 const int slot = st.slot_busy[0] ? 1 : 0;
                 const int64_t count = std::min<int64_t>(S->cfg.chunk_voxels, r->nvox - r->launched);
                 hipEvent_t ev = nullptr;
@@ -589,6 +671,8 @@ int dmi_server_wait(dmi_server *S, int64_t id, dmi_request_info *info) {
         info->voxels = std::max<int64_t>(r->nvox, 0);
         info->queue_ms = r->t_admit.time_since_epoch().count() ? ms_between(r->t_submit, r->t_admit) : 0.0;
         info->latency_ms = ms_between(r->t_submit, r->t_done);
+        info->workgroups_per_cu = r->per_cu;
+        info->trial_voxels = r->trial;
     }
     // the request is returned: forget it (its error text stays available for a while)
     const int status = r->status;

@@ -78,8 +78,8 @@ class ServerConfigC(C.Structure):
 
 
 class RequestInfoC(C.Structure):
-    _fields_ = [("status", C.c_int32), ("_pad", C.c_int32), ("voxels", C.c_int64), ("queue_ms", C.c_double),
-                ("latency_ms", C.c_double)]
+    _fields_ = [("status", C.c_int32), ("workgroups_per_cu", C.c_int32), ("voxels", C.c_int64),
+                ("queue_ms", C.c_double), ("latency_ms", C.c_double), ("trial_voxels", C.c_int64)]
 
 
 class ServerStatsC(C.Structure):

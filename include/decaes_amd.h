@@ -173,10 +173,11 @@ typedef struct {
 
 typedef struct {
     int32_t status;           /* DMI_OK or the request's error code (dmi_server_request_error has the text) */
-    int32_t _pad;
+    int32_t workgroups_per_cu; /* launch schedule of the request's chunks (dmi_stats.workgroups_per_cu) */
     int64_t voxels;           /* voxels above Threshold */
     double queue_ms;          /* submit -> admitted */
     double latency_ms;        /* submit -> outputs in host memory */
+    int64_t trial_voxels;     /* voxels run as schedule trials (the first request of sizes with none measured) */
 } dmi_request_info;
 
 typedef struct {

@@ -149,3 +149,34 @@ def test_server_full_size_request(dmi, oracle):
     oo = oracle.make_t2map_opts(shape, m, TE, n, T2R, "gcv", nRefAngles=32, nRefAnglesMin=opts.nRefAnglesMin)
     omaps, odist, sel, spread = PU.oracle_ensemble(oracle, img, oo, T2R, vox=vox)
     PU.check("server_full_size_gcv", oracle, T2R, sel, spread, maps, dist, omaps, odist)
+
+
+def test_server_measures_the_schedule(dmi):
+    """The first request of sizes with no measured launch schedule runs its first voxels once with each candidate
+    schedule and keeps the fastest (as the one-call path does): its results are bit-identical to the same request run
+    alone, and a later one-call run of the same sizes uses the schedule the request loop stored."""
+    from decaes_amd.server import Server
+    print("[server] schedule measurement", flush=True)
+    shape, m, TE, n, T2R = (120, 120, 4), 56, 7.9e-3, 60, (8e-3, 2.0)
+    img, _ = dmi.synth_mse(shape, m, TE, ncomp=4, mask_fraction=0.9, seed=77)
+    # chi2 at these sizes: never run elsewhere in the test suite with enough voxels to be measured
+    opts = dmi.T2mapOptions.from_image(img, TE=TE, nT2=n, T2Range=T2R, Reg="chi2", Chi2Factor=1.03, nRefAngles=48,
+                                       Silent=True, SaveRegParam=True)
+    popts = dmi.T2partOptions.from_t2map(opts, SPWin=(T2R[0], 40e-3), MPWin=(40e-3, T2R[1]))
+    with Server() as srv:
+        h = srv.submit(img, opts, popts)
+        sm, sd, sp, info = srv.wait(h)
+    assert info["voxels"] >= 2 * 8 * 256 * 9  # large enough for the trials
+    print(f"[server] request: {info['trial_voxels']} trial voxels, {info['workgroups_per_cu']} workgroups per CU",
+          flush=True)
+    assert info["trial_voxels"] > 0
+    am, ad, ap = _alone(dmi, img, opts, popts)
+    st = dmi.api._lib.context(None).stats()
+    _same(sd, ad)
+    for k in ("gdn", "ggm", "gva", "fnr", "snr", "alpha", "mu", "chi2factor"):
+        _same(sm[k], am[k])
+    for k in ("sfr", "sgm", "mfr", "mgm"):
+        _same(sp[k], ap[k])
+    print(f"[server] one-call run after the request loop: {st['lds_rows']} rows in LDS, "
+          f"{st['workgroups_per_cu']} workgroups per CU", flush=True)
+    assert st["workgroups_per_cu"] == info["workgroups_per_cu"]
