@@ -96,7 +96,8 @@ struct T2partParams {
 };
 
 int t2map_lda(int m, int n);
-// K: flip-angle grid size of the launch (1 with SetFlipAngle); legacy adds the spline knots and coefficients
+// LDS of one voxel workgroup; K (the flip-angle grid size of the launch) takes no LDS; legacy adds the spline knots
+// and coefficients
 size_t t2map_lds_bytes(int m, int n, int lda, int K, int legacy);
 int64_t t2map_slot_doubles(int m, int n, int gld);
 

@@ -28,7 +28,6 @@ namespace {
 
 // Explicit address spaces: pointers kept in structs otherwise decay to generic (flat) addresses.
 typedef __attribute__((address_space(3))) double ldbl;
-typedef __attribute__((address_space(3))) int lint;
 typedef __attribute__((address_space(1))) double gdbl;
 typedef const __attribute__((address_space(1))) double cgdbl;
 typedef const __attribute__((address_space(1))) int64_t cgi64;
@@ -544,9 +543,14 @@ struct Wave {
     // LDS. A is column-major with column stride lda (even, lda = 2 mod 4, lda >= mw rounded up to even):
     // a lane that owns a column streams it with 16-byte loads, and 16-byte loads of 16 consecutive lanes
     // at stride lda hit distinct banks. Every LDS double array starts 16-byte aligned.
-    ldbl *A, *b, *zz, *x, *xf, *b0, *cmu, *crn, *csemi, *cx;
+    ldbl *A, *b, *zz, *x, *cmu, *crn, *csemi;  // zz: the warm start's residual, kept in b's storage
+    // in the workgroup's global slot; every lane reads and writes only its own entries (rows L and L + 64 of b0,
+    // entry L of xf), except the fit of save_results, which reads all of xf after a workgroup barrier
+    gdbl *xf, *b0;
+    gdbl *cx;         // the cached Tikhonov solutions, n per entry, in the workgroup's global slot (each lane stores
+                      // and later loads only its own entries)
     ldbl *spt, *spc;  // legacy = true: spline knots (kSplineKnots) and coefficients (kSplineMaxPts)
-    lint *idx, *diag, *stk;
+    int idxl;         // the column at position L when the last solve ended (the reference's idx[L])
     int lda, mw, m, n, L;
     // Rows [0, rl) of the workspace are in LDS (column stride lda); rows [rl, mw) of column j, when rl < mw, are in
     // this workgroup's global slot at G[j * gld + (r - rl)] (the split schedule, which lets more workgroups share a
@@ -862,7 +866,8 @@ struct Wave {
         if (k <= 0) return zr;
         const double aL = (L < k) ? Aat(L, L) : 1.0;
         const double rL = 1.0 / aL;
-        // rows L < 64 of any column j < n stay inside the LDS allocation (b, zz, x, w follow the matrix)
+        // rows L < 64 of any column j < n stay inside the LDS allocation (b and x follow the matrix, and the
+        // allocation is padded to (n - 1) lda + 64 doubles at least)
         // volatile: keeps each load at its place in the pipeline instead of being moved next to its use
         volatile ldbl *Av = A;
         // three column registers in rotation, unrolled so that no register is copied: the column of step j - 3
@@ -1073,8 +1078,7 @@ struct Wave {
         }
         if (L < n) {
             x[pidx] = (L < nsetp) ? xp : 0.0;
-            idx[L] = pidx;
-            diag[L] = (int)((dmask >> L) & 1ull);
+            idxl = pidx;
         }
         lds_order();
         const int Mlim = tikh ? M : mr;
@@ -1147,11 +1151,11 @@ struct Wave {
             if (i < m) wj += cj[i] * zz[i];
             if (L < n - 1) { wl = wj; pos = !(wj <= 0); }
             x[L] = 0.0;
-            idx[L] = L;
-            diag[L] = 0;
+            idxl = L;
         }
         unsigned long long bal = ballot(pos);
         if (L == n - 1) wl = (bal == 0) ? 1.0 : 0.0;
+        lds_order();  // every lane has read zz before b (the same storage) takes the right-hand side
         for (int i = L; i < m; i += kWave) b[i] = b0[i];
         if (tikh)
             for (int i = m + L; i < M; i += kWave) b[i] = 0.0;
@@ -1168,7 +1172,7 @@ struct Wave {
 
     // ------------------------- Tikhonov cache (src/lsqnonneg.jl:392-444) -------------------------
     // A hit needs exactly equal mu; re-solving the same mu reproduces the same bits, so the cache is a pure
-    // memo. Entries keep mu, rnorm, ||x||^2 and x (all in LDS).
+    // memo. Entries keep mu, rnorm and ||x||^2 in LDS and x in the global slot.
     __device__ __forceinline__ void cache_reset() {
         if (L < kTikhCache) cmu[L] = NAN;
         __syncthreads();
@@ -1186,7 +1190,7 @@ struct Wave {
         int ns;
         double rn;
         solve(A0, true, mu, ns, rn);
-        double v = (L < ns) ? x[idx[L]] : 0.0;
+        double v = (L < ns) ? x[idxl] : 0.0;
         double semi = wsum(v * v);
         if (L < n) cx[slot * n + L] = x[L];
         if (L == 0) { cmu[slot] = mu; crn[slot] = rn; csemi[slot] = semi; }
@@ -1736,7 +1740,9 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
     W.flops += 2ull * N * N * (m >= n ? m : n);
     lds_order();
     DMI_LAP(kPhGram);
-    ldbl *dv = W.b, *ev = W.b + N, *vv = W.zz, *qv = W.zz + N;
+    // the tridiagonal in b (2N <= m + n entries), the reflector in x (N <= n); the vector q of the rank-2 update
+    // stays in the lanes' registers and is broadcast with readlane
+    ldbl *dv = W.b, *ev = W.b + N, *vv = W.x;
     for (int k = 0; k < N - 2; ++k) {
         double xr = (L > k && L < N) ? G(L, k) : 0.0;
         double nrm = sqrt(wsum(xr * xr));
@@ -1780,21 +1786,20 @@ __device__ __forceinline__ void gram_eigs(Wave &W, double &lam_lane, int &N) {
         }
         double Kq = wsum(vr * pr);
         double qr = pr - Kq * vr;
-        if (L < N) qv[L] = qr;
         lds_order();
-        // rank-2 update, four columns per step with every load of the step ahead of its stores (the loads of
-        // qv / vv could alias the stores to G as far as the compiler knows, so a plain loop would serialize)
+        // rank-2 update, four columns per step with every load of the step ahead of its stores (the loads of vv
+        // could alias the stores to G as far as the compiler knows, so a plain loop would serialize)
         if (L > k && L < N) {
             int c = k + 1;
             for (; c + 4 <= N; c += 4) {
                 double g[4], q[4], v[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) { g[r] = G(L, c + r); q[r] = qv[c + r]; v[r] = vv[c + r]; }
+                for (int r = 0; r < 4; ++r) { g[r] = G(L, c + r); q[r] = readlane(qr, c + r); v[r] = vv[c + r]; }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) G(L, c + r) = g[r] - 2.0 * (vr * q[r] + qr * v[r]);
             }
-            for (; c < N; ++c) G(L, c) -= 2.0 * (vr * qv[c] + qr * vv[c]);
+            for (; c < N; ++c) G(L, c) -= 2.0 * (vr * readlane(qr, c) + qr * vv[c]);
         }
         W.flops += 8ull * (N - k) * (N - k);
         lds_order();
@@ -2356,26 +2361,29 @@ struct FlipSearch {
     __device__ __forceinline__ double run() {
         seen = 0ull;
         numeval = 0;
-        // initialize! (:716-734): depth-limited pre-order bisection, explicit stack
+        // initialize! (:716-734): depth-limited pre-order bisection, walked over the paths of the box tree (bit k of
+        // path: the half taken at level k + 1, 0 left, 1 right) instead of a stack. A box at level len is evaluated,
+        // and its children (len + 1 < depth) are visited only if fewer than mineval points have been evaluated;
+        // the boxes already due (right siblings of the path) are evaluated either way, as the recursion does.
         for (int depth = 1; depth <= mineval; ++depth) {
-            int sp = 0;
-            if (W.L == 0) { W.stk[0] = 0; W.stk[1] = K - 1; W.stk[2] = depth; }
-            sp = 1;
-            lds_order();
-            while (sp > 0) {
-                sp -= 1;
-                int lo = rfl(W.stk[3 * sp]), hi = rfl(W.stk[3 * sp + 1]), dp = rfl(W.stk[3 * sp + 2]);
-                lds_order();
-                if (dp <= 0) continue;
-                evaluate_box(lo, hi, false, 0.0);
-                if (numeval >= mineval) continue;
-                int mid = (lo + hi) / 2;
-                if (W.L == 0) {
-                    W.stk[3 * sp] = mid; W.stk[3 * sp + 1] = hi; W.stk[3 * sp + 2] = dp - 1;
-                    W.stk[3 * sp + 3] = lo; W.stk[3 * sp + 4] = mid; W.stk[3 * sp + 5] = dp - 1;
+            unsigned long long path = 0ull;
+            int len = 0;
+            for (;;) {
+                int lo = 0, hi = K - 1;
+                for (int b = 0; b < len; ++b) {
+                    const int mid = (lo + hi) / 2;
+                    if ((path >> b) & 1ull) lo = mid;
+                    else hi = mid;
                 }
-                sp += 2;
-                lds_order();
+                evaluate_box(lo, hi, false, 0.0);
+                if (numeval < mineval && len + 1 < depth) {  // left child
+                    path &= ~(1ull << len);
+                    ++len;
+                    continue;
+                }
+                while (len > 0 && ((path >> (len - 1)) & 1ull)) --len;  // leave the finished right halves
+                if (len == 0) break;
+                path |= 1ull << (len - 1);  // the right half after the left one
             }
             if (numeval >= mineval) break;
         }
@@ -2476,27 +2484,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     const int bz = (p.mw + 1) & ~1;
     W.A = s; s += (size_t)p.n * p.lda;
     W.b = s; s += bz;
-    W.zz = s; s += bz;
+    W.zz = W.b;  // the warm start forms its residual there before b takes the right-hand side
     const int ne = (p.n + 1) & ~1;
     W.x = s; s += ne;
-    W.xf = s; s += ne;
-    W.b0 = s; s += (p.m + 1) & ~1;
     W.cmu = s; s += kTikhCache;
     W.crn = s; s += kTikhCache;
     W.csemi = s; s += kTikhCache;
-    W.cx = s; s += (size_t)kTikhCache * p.n;  // the cached Tikhonov solutions, n per entry
     W.spt = s; s += LEGACY ? kSplineKnots : 0;
     W.spc = s; s += LEGACY ? kSplineMaxPts : 0;
-    lint *si = (lint *)s;
-    W.idx = si; si += ne;
-    W.diag = si; si += ne;
-    W.stk = si;  // 3 (K + 2) ints: the depth-limited bisection stack holds at most nRefAnglesMin + 1 <= K + 1 boxes
     gdbl *slot = (gdbl *)p.scratch + (int64_t)blockIdx.x * p.slot_stride;
-    // slot: the voxel's column-major basis (one copy, so the slots of the resident workgroups stay within L2) and the
-    // L-curve cache
+    // slot: the voxel's column-major basis (one copy), the L-curve caches, the Tikhonov cache solutions, the
+    // normalized signal, the final distribution and the workspace rows of the split schedule
     W.A0s = slot;
     W.lc = slot + (int64_t)basis_ld(p.m) * p.n;
-    W.G = W.lc + 7 * kLcurveCap;  // workspace rows [rl, mw) of the split schedule, n columns of stride gld
+    W.cx = W.lc + 7 * kLcurveCap;
+    W.b0 = W.cx + (size_t)kTikhCache * p.n;  // the voxel's normalized signal
+    W.xf = W.b0 + ((p.m + 1) & ~1);           // the voxel's final distribution
+    W.G = W.xf + ne;  // workspace rows [rl, mw) of the split schedule, n columns of stride gld
     W.flops = 0;
     W.flops32 = 0;
     W.solves = 0;
@@ -2675,16 +2679,19 @@ int t2map_lda(int m, int n) {
 }
 
 size_t t2map_lds_bytes(int m, int n, int lda, int K, int legacy) {
+    (void)K;
     const size_t ne = (size_t)((n + 1) & ~1), bz = (size_t)((m + n + 1) & ~1);
-    size_t d = (size_t)n * lda + 2 * bz + 2 * ne + (size_t)((m + 1) & ~1) + 3 * kTikhCache +
-               (size_t)kTikhCache * n + (legacy ? (size_t)kSplineLds : 0);
-    size_t i = 2 * ne + 3 * ((size_t)K + 2);
-    return d * sizeof(double) + i * sizeof(int);
+    size_t d = (size_t)n * lda + bz + ne + 3 * kTikhCache + (legacy ? (size_t)kSplineLds : 0);
+    // tri_solve reads rows L < 64 of every column: at least (n - 1) lda + 64 doubles
+    d = std::max(d, (size_t)(n - 1) * lda + 64);
+    return d * sizeof(double);
 }
 
 int64_t t2map_slot_doubles(int m, int n, int gld) {
-    // basis, function cache (3), point cache (4), the workspace rows kept outside LDS (split schedule)
-    int64_t s = (int64_t)basis_ld(m) * n + 7 * (int64_t)kLcurveCap + (int64_t)n * gld;
+    // basis, L-curve function cache (3) and point cache (4), Tikhonov cache solutions, normalized signal, final
+    // distribution, the workspace rows kept outside LDS (split schedule)
+    int64_t s = (int64_t)basis_ld(m) * n + 7 * (int64_t)kLcurveCap + (int64_t)kTikhCache * n + ((m + 1) & ~1) +
+                ((n + 1) & ~1) + (int64_t)n * gld;
     return (s + 31) / 32 * 32;
 }
 
