@@ -4,11 +4,11 @@
         --SPWin 10e-3 40e-3 --MPWin 40e-3 2.0 --Reg lcurve -o results/
     python -m decaes_amd.cli @settings.txt          # one argument per line, as the reference reads settings files
 
-For each input file: load the 4D image (MAT or NIfTI), multiply by the mask file if given, run T2mapSEcorr (with the
-B1 map if given) and/or T2partSEcorr, and write <name>.t2dist.mat, <name>.t2maps.mat, <name>.t2parts.mat and
-<name>.log into the output folder (MAT v7.3, as the reference's MAT.jl writes them), copying each @settings file there
-as <name>.<settings file name>. A failure on one file is logged and the next file is processed. --bet runs the FSL BET
-executable given by --betpath, as the reference does; PAR/REC inputs are not available in this build.
+For each input file: load the 4D image (MAT, NIfTI, PAR/REC or XML/REC), multiply by the mask file if given, run
+T2mapSEcorr (with the B1 map if given) and/or T2partSEcorr, and write <name>.t2dist.mat, <name>.t2maps.mat,
+<name>.t2parts.mat and <name>.log into the output folder (MAT v7.3, as the reference's MAT.jl writes them), copying
+each @settings file there as <name>.<settings file name>. A failure on one file is logged and the next file is
+processed. --bet runs the FSL BET executable given by --betpath, as the reference does.
 """
 from __future__ import annotations
 

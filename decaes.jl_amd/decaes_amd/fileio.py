@@ -5,7 +5,8 @@
   a warning. Inputs: v7.3 files through libhdf5, v4/v5/v7 files through scipy.io.
 * NIfTI-1 (.nii, .nii.gz): a reader and writer of the single-file format, with scl_slope / scl_inter applied as the
   reference does (slope 0 means unscaled).
-* PAR/REC: not supported by this build.
+* PAR/REC and XML/REC (.par, .xml, .rec): Philips' formats, read by decaes_amd.parrec and scaled to floating-point
+  values.
 """
 from __future__ import annotations
 
@@ -105,8 +106,8 @@ def ensure_ndims(filename, data: np.ndarray, n: int) -> np.ndarray:
 
 
 def load_image(filename, ndims: int = 4) -> np.ndarray:
-    """load_image (src/main.jl:577-621): the first `ndims`-D array of a MAT file, or a NIfTI image; Float64,
-    Fortran order."""
+    """load_image (src/main.jl:577-621): the first `ndims`-D array of a MAT file, a NIfTI image or a PAR/REC or
+    XML/REC image; Float64, Fortran order."""
     ext = maybe_get_suffix(filename)
     if ext == ".mat":
         data = read_mat(filename)
@@ -121,7 +122,9 @@ def load_image(filename, ndims: int = 4) -> np.ndarray:
     elif ext in (".nii", ".nii.gz"):
         arr = read_nifti(filename)
     elif ext in (".par", ".xml", ".rec"):
-        raise NotImplementedError(f"{filename}: PAR/REC input is not supported by this build")
+        from . import parrec
+
+        arr = parrec.load(filename)
     else:
         raise ValueError(f"Currently, only {', '.join(ALLOWED_FILE_SUFFIXES)} files are supported")
     arr = ensure_ndims(filename, np.asarray(arr), ndims)

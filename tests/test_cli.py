@@ -64,7 +64,8 @@ def test_load_image_mat_and_ndims(tmp_path):
     with pytest.raises(ValueError, match="No 4-D array"):
         scipy.io.savemat(tmp_path / "none.mat", {"v": np.ones(3)})
         fileio.load_image(tmp_path / "none.mat", 4)
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(FileNotFoundError, match="no .par or .xml"):
+        (tmp_path / "x.rec").write_bytes(b"")
         fileio.load_image(tmp_path / "x.rec", 4)
 
 

@@ -15,7 +15,7 @@ def _image(dmi, seed=3):
     return img
 
 
-@pytest.mark.parametrize("fmt,settings", [(".mat", False), ("v7.3 .mat", False), (".nii.gz", True)])
+@pytest.mark.parametrize("fmt,settings", [(".mat", False), ("v7.3 .mat", False), (".nii.gz", True), (".par", False)])
 def test_cli_matches_api(dmi, tmp_path, fmt, settings):
     from decaes_amd import cli, fileio, mat73
     print(f"[cli] {fmt} settings={settings}", flush=True)
@@ -25,6 +25,13 @@ def test_cli_matches_api(dmi, tmp_path, fmt, settings):
         scipy.io.savemat(inp, {"img": img})
     elif fmt == "v7.3 .mat":
         mat73.write(inp, {"img": img})
+    elif fmt == ".par":
+        # 16-bit pixel values with a power-of-two scale slope, so the floating-point values are exact
+        import parrec_files as PF
+        step = 2.0 ** np.ceil(np.log2(img.max() / 60000.0))
+        pv = np.round(img / step).astype(np.uint16)
+        img = pv.astype(np.float64) * step
+        PF.write_rec(tmp_path / "scan.rec", PF.write_par(inp, pv[..., None], 1.0, 0.0, 1.0 / step))
     else:
         fileio.write_nifti(inp, img)
     out = tmp_path / "out"
