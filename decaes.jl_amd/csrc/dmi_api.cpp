@@ -22,7 +22,8 @@ thread_local std::string g_create_error;
 struct dmi_ctx {
     int device = 0;
     int num_cu = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // the stream every call runs on: own_stream, or the caller's (dmi_set_stream)
+    hipStream_t own_stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::string err;
     std::mutex mu;
@@ -35,6 +36,7 @@ struct dmi_ctx {
     DevBuf img, ain, out[12], t2p[5];
     DevBuf f32;
     DevBuf weights, tlog;
+    std::vector<double> tlog_host, weights_host;  // the T2part tables last uploaded to tlog / weights
 };
 
 namespace {
@@ -46,6 +48,26 @@ int fail(dmi_ctx *c, int code, const char *msg) {
 }
 
 #define HIPCHECK(c, expr) DMI_HIP((c)->err, expr)
+
+// Whether ctx's stream is being captured into a HIP graph: calls then record their launches without host reads,
+// waits, events or allocations.
+int capturing(dmi_ctx *c, bool &rec) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHECK(c, hipStreamIsCapturing(c->stream, &cs));
+    rec = cs != hipStreamCaptureStatusNone;
+    return DMI_OK;
+}
+
+// Entry points on host arrays copy through pageable memory and wait: refused while the stream is captured.
+int no_capture(dmi_ctx *c, const char *name) {
+    bool rec = false;
+    int rc = capturing(c, rec);
+    if (rc) return rc;
+    if (rec)
+        return dmi_host::errorf(c->err, DMI_EINVAL, "%s copies host arrays and waits, which graph capture does not "
+                                "allow: use the device-array entry points", name);
+    return DMI_OK;
+}
 
 // Device memory one T2mapSEcorr call works in: the grids of its options, a voxel list of vox_cap entries, the
 // launch counters and the per-workgroup slots of the voxel kernel.
@@ -154,11 +176,49 @@ int run_t2map_on(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, con
     return DMI_OK;
 }
 
+// run_t2map_on recorded into the HIP graph being captured on ctx's stream: the same launches without a host read,
+// wait or event. Each chunk's count of selected voxels stays on the device (counter[0], read by the voxel kernel,
+// which takes its work from counter[1]); the schedule is the one measured for these sizes, or the default, and the
+// persistent grid is sized for the whole chunk (workgroups beyond the selected voxels end at once). Statistics are
+// not filled.
+int record_t2map_on(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, const double *d_alpha_in,
+                    const dmi_t2map_out *d, const RunBufs &rb) {
+    const int m = o->nTE, n = o->nT2;
+    const int64_t V = o->MatrixSize[0] * o->MatrixSize[1] * o->MatrixSize[2];
+    if (!std::isnan(o->SetFlipAngle) && o->SaveNNLSBasis)
+        HIPCHECK(c, hipMemcpy2DAsync(d->decaybasis, sizeof(double) * m, rb.g.AsT, sizeof(double) * dmi::basis_ld(m),
+                                     sizeof(double) * m, n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(c, dmi::launch_fill(rb.counters, sizeof(unsigned long long) * 4, 0u, c->stream));
+    HIPCHECK(c, dmi::launch_fill(rb.status, sizeof(int) * 4, 0u, c->stream));
+    c->stats = dmi_stats{};
+    dmi::T2mapParams p = dmi_host::t2map_params(o, rb.g, d_image, d_alpha_in, d);
+    dmi_host::Schedule sched;
+    int rc = dmi_host::t2map_schedule(c->err, o, sched);
+    if (rc) return rc;
+    dmi_host::apply_schedule(p, sched);
+    unsigned *count = rb.counter, *work = rb.counter + 1;
+    p.nvox_dev = count;
+    for (int64_t v0 = 0; v0 < V; v0 += rb.vox_cap) {
+        const int64_t v1 = std::min<int64_t>(V, v0 + rb.vox_cap);
+        HIPCHECK(c, dmi::launch_fill(count, sizeof(unsigned), 0u, c->stream));
+        HIPCHECK(c, dmi::launch_select(d_image, v0, v1, o->Threshold, rb.vox, count, c->stream));
+        rc = dmi_host::t2map_launch_raw(c->err, p, rb.vox, v1 - v0, work, rb.counters, rb.status, rb.slots,
+                                        rb.slot_doubles, c->num_cu, c->stream);
+        if (rc) return rc;
+    }
+    return DMI_OK;
+}
+
 int run_t2map(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, const double *d_alpha_in,
               const dmi_t2map_out *d) {
     const int64_t V = o->MatrixSize[0] * o->MatrixSize[1] * o->MatrixSize[2];
     int rc = dmi_host::validate_outputs(c->err, o, d);
     if (rc) return rc;
+    bool rec = false;
+    if ((rc = capturing(c, rec))) return rc;
+    if (rec)
+        return dmi_host::errorf(c->err, DMI_EINVAL, "this entry point allocates and waits, which graph capture does "
+                                "not allow: use dmi_t2map_f64_device_noalloc");
     HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
     rc = c->grids.ensure(c->err, o, c->stream);
     if (rc) return rc;
@@ -212,19 +272,25 @@ int run_t2map_noalloc(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o
     rb.vox_cap = dmi::kNoAllocVox;
     rb.slots = S + dmi::kNoAllocOffSlots;
     rb.slot_doubles = dmi::kNoAllocSlots;
-    HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+    bool rec = false;
+    if ((rc = capturing(c, rec))) return rc;
+    if (!rec) HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
     {
         std::lock_guard<std::mutex> lk(g_noalloc_mu);
         const dmi_host::GridKey k = dmi_host::GridKey::of(o);
         dmi_host::GridKey &have = g_noalloc_key[c->device];
         if (!(have == k)) {
+            // building the grids copies host tables, which a captured graph would read again at every replay
+            if (rec)
+                return dmi_host::errorf(c->err, DMI_EINVAL, "under graph capture the grids of these options must "
+                                        "already be in the static scratch: make one call with them outside capture first");
             have = dmi_host::GridKey{};
             rc = dmi_host::build_grids(c->err, o, rb.g, c->stream);
             if (rc) return rc;
             have = k;
         }
     }
-    return run_t2map_on(c, d_image, o, d_alpha_in, d, rb);
+    return rec ? record_t2map_on(c, d_image, o, d_alpha_in, d, rb) : run_t2map_on(c, d_image, o, d_alpha_in, d, rb);
 }
 
 int run_t2part(dmi_ctx *c, const double *d_dist, const dmi_t2part_opts *o, double *sfr, double *sgm, double *mfr,
@@ -233,18 +299,33 @@ int run_t2part(dmi_ctx *c, const double *d_dist, const dmi_t2part_opts *o, doubl
     dmi::T2partParams p{};
     int rc = dmi_host::t2part_prepare(c->err, o, lg, wts, p);
     if (rc) return rc;
-    HIPCHECK(c, c->tlog.ensure(sizeof(double) * p.n));
-    HIPCHECK(c, hipMemcpyAsync(c->tlog.p, lg.data(), sizeof(double) * p.n, hipMemcpyHostToDevice, c->stream));
-    p.logT2 = c->tlog.as<double>();
-    p.weights = nullptr;
-    if (!wts.empty()) {
-        HIPCHECK(c, c->weights.ensure(sizeof(double) * p.n));
-        HIPCHECK(c, hipMemcpyAsync(c->weights.p, wts.data(), sizeof(double) * p.n, hipMemcpyHostToDevice, c->stream));
-        p.weights = c->weights.as<double>();
+    bool rec = false;
+    if ((rc = capturing(c, rec))) return rc;
+    if (rec) {
+        // a captured graph reads the tables uploaded by an earlier call with the same options
+        if (lg != c->tlog_host || wts != c->weights_host)
+            return dmi_host::errorf(c->err, DMI_EINVAL, "under graph capture the T2 tables of these options must "
+                                    "already be on the device: make one call with them outside capture first");
+    } else {
+        c->tlog_host.clear();
+        c->weights_host.clear();
+        HIPCHECK(c, c->tlog.ensure(sizeof(double) * p.n));
+        HIPCHECK(c, hipMemcpyAsync(c->tlog.p, lg.data(), sizeof(double) * p.n, hipMemcpyHostToDevice, c->stream));
+        if (!wts.empty()) {
+            HIPCHECK(c, c->weights.ensure(sizeof(double) * p.n));
+            HIPCHECK(c, hipMemcpyAsync(c->weights.p, wts.data(), sizeof(double) * p.n, hipMemcpyHostToDevice,
+                                       c->stream));
+        }
     }
+    p.logT2 = c->tlog.as<double>();
+    p.weights = wts.empty() ? nullptr : c->weights.as<double>();
     p.dist = d_dist; p.sfr = sfr; p.sgm = sgm; p.mfr = mfr; p.mgm = mgm;
     HIPCHECK(c, dmi::launch_t2part(p, c->stream));
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (!rec) {
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        c->tlog_host = lg;
+        c->weights_host = wts;
+    }
     return DMI_OK;
 }
 
@@ -275,6 +356,7 @@ int dmi_create(dmi_ctx **out, int device, uint32_t flags) {
         delete c;
         return dmi_host::errorf(g_create_error, DMI_EHIP, "stream creation failed: %s", hipGetErrorString(e));
     }
+    c->own_stream = c->stream;
     for (auto &ev : c->ev) (void)hipEventCreate(&ev);
     if ((e = dmi::noalloc_scratch(device, &c->noalloc)) != hipSuccess) {
         for (auto &ev : c->ev) (void)hipEventDestroy(ev);
@@ -289,7 +371,7 @@ int dmi_create(dmi_ctx **out, int device, uint32_t flags) {
 int dmi_destroy(dmi_ctx *c) {
     if (!c) return DMI_OK;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
     c->grids.release();
     c->lb.release();
     DevBuf *bufs[] = {&c->vox, &c->img, &c->ain, &c->weights, &c->tlog, &c->f32};
@@ -298,8 +380,24 @@ int dmi_destroy(dmi_ctx *c) {
     for (auto &b : c->t2p) b.release();
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
+    return DMI_OK;
+}
+
+int dmi_set_stream(dmi_ctx *c, void *stream) {
+    if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!stream) {
+        c->stream = c->own_stream;
+        return DMI_OK;
+    }
+    hipDevice_t dev = -1;
+    HIPCHECK(c, hipStreamGetDevice(static_cast<hipStream_t>(stream), &dev));
+    if (dev != c->device)
+        return dmi_host::errorf(c->err, DMI_EINVAL, "the stream is on device %d, the context on device %d", (int)dev,
+                                c->device);
+    c->stream = static_cast<hipStream_t>(stream);
     return DMI_OK;
 }
 
@@ -338,6 +436,7 @@ int dmi_t2map_f64(dmi_ctx *c, const double *image, const dmi_t2map_opts *o, cons
     if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHECK(c, hipSetDevice(c->device));
+    if (int rc0 = no_capture(c, "dmi_t2map_f64")) return rc0;
     int rc = dmi_host::validate_t2map(c->err, o);
     if (rc) return rc;
     if (!image || !out) return fail(c, DMI_EINVAL, "image and out are required");
@@ -398,6 +497,7 @@ int dmi_t2map_f32(dmi_ctx *c, const float *image, const dmi_t2map_opts *o, const
     if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHECK(c, hipSetDevice(c->device));
+    if (int rc0 = no_capture(c, "dmi_t2map_f32")) return rc0;
     int rc = dmi_host::validate_t2map(c->err, o);
     if (rc) return rc;
     if (!image || !out) return fail(c, DMI_EINVAL, "image and out are required");
@@ -434,6 +534,7 @@ int dmi_t2part_f32(dmi_ctx *c, const float *dist, const dmi_t2part_opts *o, floa
     if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHECK(c, hipSetDevice(c->device));
+    if (int rc0 = no_capture(c, "dmi_t2part_f32")) return rc0;
     if (!o) return fail(c, DMI_EINVAL, "opts is NULL");
     if (!dist || !sfr || !sgm || !mfr || !mgm) return fail(c, DMI_EINVAL, "dist and all four outputs are required");
     const int64_t V = o->MatrixSize[0] * o->MatrixSize[1] * o->MatrixSize[2];
@@ -454,6 +555,7 @@ int dmi_epg_basis_set(dmi_ctx *c, const dmi_t2map_opts *o, double *out) {
     if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHECK(c, hipSetDevice(c->device));
+    if (int rc0 = no_capture(c, "dmi_epg_basis_set")) return rc0;
     int rc = dmi_host::validate_t2map(c->err, o);
     if (rc) return rc;
     if (!out) return fail(c, DMI_EINVAL, "out is NULL");
@@ -484,6 +586,7 @@ int dmi_t2part_f64(dmi_ctx *c, const double *dist, const dmi_t2part_opts *o, dou
     if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHECK(c, hipSetDevice(c->device));
+    if (int rc0 = no_capture(c, "dmi_t2part_f64")) return rc0;
     if (!o) return fail(c, DMI_EINVAL, "opts is NULL");
     if (!dist || !sfr || !sgm || !mfr || !mgm) return fail(c, DMI_EINVAL, "dist and all four outputs are required");
     const int64_t V = o->MatrixSize[0] * o->MatrixSize[1] * o->MatrixSize[2];

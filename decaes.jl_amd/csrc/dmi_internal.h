@@ -66,7 +66,8 @@ struct T2mapParams {
     int save_resnorm, save_decaycurve, save_regparam, save_basis;
     int legacy;           // legacy = true: spline surrogate for the flip angle, chi2 doubling search
     int64_t V;            // voxels in the volume
-    int64_t nvox;         // voxels above threshold
+    int64_t nvox;         // voxels above threshold (with nvox_dev: an upper bound, the length of the voxel list)
+    const unsigned int *nvox_dev;  // or nullptr; else the count of voxels above threshold, read on the device
     double E1, beta, chi2factor, noiselevel, setflip;
     const double *image;  // (V, m) column-major
     const int64_t *vox;   // nvox linear indices

@@ -2510,11 +2510,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     DMI_PHASE_BEGIN();
     const int m = p.m, n = p.n, L = W.L;
     const int64_t V = p.V;
+    // the count of selected voxels: a launch parameter, or (for launches recorded into a graph) the selection
+    // kernel's counter, which never exceeds the list length p.nvox
+    const int64_t nvox = p.nvox_dev ? (int64_t)min((unsigned long long)*p.nvox_dev, (unsigned long long)p.nvox) : p.nvox;
     for (;;) {
         unsigned int vi = 0;
         if (L == 0) vi = atomicAdd(p.counter, 1u);
         vi = (unsigned)rfl((int)__shfl((int)vi, 0, kWave));
-        if ((int64_t)vi >= p.nvox) break;
+        if ((int64_t)vi >= nvox) break;
         const int64_t I = ((cgi64 *)p.vox)[vi];
         W.failed = false;
         // voxelwise_T2_distribution! (:201-238): normalize by the max echo

@@ -28,7 +28,8 @@ _dp = C.POINTER(C.c_double)
 
 # Every symbol include/decaes_amd.h declares.
 EXPORTS = (
-    "dmi_create", "dmi_destroy", "dmi_last_error", "dmi_t2map_f64", "dmi_t2map_f64_device", "dmi_epg_basis_set",
+    "dmi_create", "dmi_destroy", "dmi_set_stream", "dmi_last_error", "dmi_t2map_f64", "dmi_t2map_f64_device",
+    "dmi_epg_basis_set",
     "dmi_t2part_f64", "dmi_t2part_f64_device", "dmi_last_stats", "dmi_synth_mse",
     "dmi_t2map_f32", "dmi_t2part_f32", "dmi_t2map_f64_device_noalloc",
     "dmi_server_create", "dmi_server_destroy", "dmi_server_submit", "dmi_server_wait", "dmi_server_poll",
@@ -105,6 +106,7 @@ def load() -> C.CDLL:
         vp = C.c_void_p
         L.dmi_create.argtypes = [C.POINTER(vp), C.c_int, C.c_uint32]
         L.dmi_destroy.argtypes = [vp]
+        L.dmi_set_stream.argtypes = [vp, vp]
         L.dmi_last_error.argtypes = [vp]
         L.dmi_last_error.restype = C.c_char_p
         L.dmi_t2map_f64.argtypes = [vp, _dp, C.POINTER(T2mapOptsC), _dp, C.POINTER(T2mapOutC)]
@@ -169,6 +171,12 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_stream(self, stream=None):
+        """Runs later calls on `stream` (a torch.cuda.Stream or a raw hipStream_t handle), or on the context's own
+        stream when None (dmi_set_stream)."""
+        h = None if stream is None else getattr(stream, "cuda_stream", stream)
+        check(load().dmi_set_stream(self.handle, C.c_void_p(h) if h else None), self.handle)
 
     def stats(self) -> dict:
         s = StatsC()

@@ -50,28 +50,69 @@ def _dptr(t):
     return C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_double))
 
 
+def _call(ctx, stream, fn, *args):
+    """fn(ctx.handle, *args) on `stream` (dmi_set_stream) when one is given, else on the context's own stream."""
+    if stream is not None:
+        ctx.set_stream(stream)
+    try:
+        rc = fn(ctx.handle, *args)
+    finally:
+        if stream is not None:
+            ctx.set_stream(None)
+    _lib.check(rc, ctx.handle)
+
+
 def t2map_device(image, opts: T2mapOptions, out: dict, alpha=None, ctx: Optional[_lib.Context] = None,
-                 noalloc: bool = False) -> dict:
+                 noalloc: bool = False, stream=None) -> dict:
     """dmi_t2map_f64_device on torch tensors (dmi_t2map_f64_device_noalloc with noalloc=True: no device allocation
-    in the call); returns the call's device counters (dmi_last_stats)."""
+    in the call); returns the call's device counters (dmi_last_stats). With `stream` (a torch.cuda.Stream) the call
+    runs on that stream, after the work queued on it; while that stream is captured into a graph, a noalloc call
+    records its launches there."""
     ctx = ctx or _lib.context(image.device.index or 0)
-    _ready(image)
+    if stream is None:
+        _ready(image)
     oc = t2map_opts_c(opts)
     oo = _lib.T2mapOutC(_dptr(out["dist"]), *(_dptr(out.get(k)) for k in (
         "gdn", "ggm", "gva", "fnr", "snr", "alpha", *_OPTIONAL)))
     L = _lib.load()
     fn = L.dmi_t2map_f64_device_noalloc if noalloc else L.dmi_t2map_f64_device
-    rc = fn(ctx.handle, C.c_void_p(image.data_ptr()), C.byref(oc),
-                                          C.c_void_p(alpha.data_ptr()) if alpha is not None else None, C.byref(oo))
-    _lib.check(rc, ctx.handle)
+    _call(ctx, stream, fn, C.c_void_p(image.data_ptr()), C.byref(oc),
+          C.c_void_p(alpha.data_ptr()) if alpha is not None else None, C.byref(oo))
     return ctx.stats()
 
 
-def t2part_device(dist, opts: T2partOptions, out: dict, ctx: Optional[_lib.Context] = None) -> None:
+def t2part_device(dist, opts: T2partOptions, out: dict, ctx: Optional[_lib.Context] = None, stream=None) -> None:
+    """dmi_t2part_f64_device on torch tensors; `stream` as for t2map_device."""
     ctx = ctx or _lib.context(dist.device.index or 0)
-    _ready(dist)
+    if stream is None:
+        _ready(dist)
     oc = t2part_opts_c(opts)
     v = C.c_void_p
-    rc = _lib.load().dmi_t2part_f64_device(ctx.handle, v(dist.data_ptr()), C.byref(oc), v(out["sfr"].data_ptr()),
-                                           v(out["sgm"].data_ptr()), v(out["mfr"].data_ptr()), v(out["mgm"].data_ptr()))
-    _lib.check(rc, ctx.handle)
+    _call(ctx, stream, _lib.load().dmi_t2part_f64_device, v(dist.data_ptr()), C.byref(oc), v(out["sfr"].data_ptr()),
+          v(out["sgm"].data_ptr()), v(out["mfr"].data_ptr()), v(out["mgm"].data_ptr()))
+
+
+def capture(image, opts: T2mapOptions, out: dict, alpha=None, t2part=None, ctx: Optional[_lib.Context] = None):
+    """T2mapSEcorr (dmi_t2map_f64_device_noalloc) and, with t2part = (T2partOptions, outputs), T2partSEcorr on its
+    distribution, recorded into a torch.cuda.CUDAGraph. Each graph.replay() fills the outputs with NaN and recomputes
+    them from the current contents of `image` (and `alpha`), as fresh calls on NaN-filled outputs would.
+
+    The calls run once outside capture first: that puts the grids of `opts` in the device's static scratch and the
+    T2part tables on the context, which the graph reads at every replay. A later call with other options on the same
+    device replaces them; call with the graph's options again before replaying."""
+    import torch
+
+    ctx = ctx or _lib.context(image.device.index or 0)
+    t2map_device(image, opts, out, alpha, ctx, noalloc=True)
+    if t2part is not None:
+        t2part_device(out["dist"], t2part[0], t2part[1], ctx)
+    torch.cuda.synchronize(image.device)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        s = torch.cuda.current_stream(image.device)
+        for t in list(out.values()) + (list(t2part[1].values()) if t2part is not None else []):
+            t.fill_(float("nan"))
+        t2map_device(image, opts, out, alpha, ctx, noalloc=True, stream=s)
+        if t2part is not None:
+            t2part_device(out["dist"], t2part[0], t2part[1], ctx, stream=s)
+    return g

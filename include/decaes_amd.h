@@ -105,6 +105,12 @@ typedef struct {
  * Replaces the per-task thread_buffer_maker (src/T2mapSEcorr.jl:596-614). flags: reserved, pass 0. */
 int dmi_create(dmi_ctx **out, int device, uint32_t flags);
 int dmi_destroy(dmi_ctx *ctx);
+/* Runs ctx's later calls on `stream` (a hipStream_t of ctx's device), or again on ctx's own stream when stream is
+ * NULL. No counterpart in the reference (a Julia task has no stream): it puts this library's kernels in the caller's
+ * stream order, and lets a caller capture them into a HIP graph with its own kernels: while `stream` is being
+ * captured, dmi_t2map_f64_device_noalloc and dmi_t2part_f64_device record their launches and return at once (see
+ * there); the other compute entry points return DMI_EINVAL. ctx never destroys a caller's stream. */
+int dmi_set_stream(dmi_ctx *ctx, void *stream);
 /* Error string of the last failed call on ctx (or of dmi_create when ctx is NULL). */
 const char *dmi_last_error(const dmi_ctx *ctx);
 
@@ -124,7 +130,14 @@ int dmi_t2map_f64_device(dmi_ctx *ctx, const double *d_image, const dmi_t2map_op
  * about 60 MB that each device holds for the life of the process; the caller passes inputs and outputs as for
  * dmi_t2map_f64_device. Limits: nTE <= 124, nT2 <= 64, nRefAngles <= 64 (DMI_ECAPACITY beyond); the volume is
  * processed in chunks of 262144 voxels, so its size is not limited. Every call on one device shares that array:
- * calls that overlap on one device (from several contexts or threads) race and must be serialized by the caller. */
+ * calls that overlap on one device (from several contexts or threads) race and must be serialized by the caller.
+ * Under HIP graph capture of ctx's stream (dmi_set_stream) the call records its launches (per chunk: voxel
+ * selection and the voxel kernel, the count of selected voxels staying on the device) and returns without waiting;
+ * every replay of the graph recomputes the outputs from the image's current contents. The grids of the options must
+ * already be in the static array (an earlier call with the same options outside capture; DMI_EINVAL otherwise), and
+ * a replay reads them there: a call with other options on the device in between rebuilds them, and the graph's
+ * results are wrong until a call with its options has rebuilt them. dmi_last_stats is not filled by a captured
+ * call. */
 int dmi_t2map_f64_device_noalloc(dmi_ctx *ctx, const double *d_image, const dmi_t2map_opts *opts,
                                  const double *d_alpha_in, const dmi_t2map_out *d_out);
 
@@ -146,7 +159,9 @@ int dmi_t2part_f32(dmi_ctx *ctx, const float *dist, const dmi_t2part_opts *opts,
 int dmi_epg_basis_set(dmi_ctx *ctx, const dmi_t2map_opts *opts, double *out);
 
 /* T2partSEcorr(dist, opts) (src/T2partSEcorr.jl:40-71): dist (X,Y,Z,nT2) -> sfr, sgm, mfr, mgm (X,Y,Z),
- * caller-allocated and NaN-filled. Host and device-pointer forms. */
+ * caller-allocated and NaN-filled. Host and device-pointer forms. Under HIP graph capture of ctx's stream the
+ * device form records its one launch, reading the T2 tables that the last call outside capture on ctx uploaded;
+ * they must be those of the same options (DMI_EINVAL otherwise). */
 int dmi_t2part_f64(dmi_ctx *ctx, const double *dist, const dmi_t2part_opts *opts, double *sfr, double *sgm,
                    double *mfr, double *mgm);
 int dmi_t2part_f64_device(dmi_ctx *ctx, const double *d_dist, const dmi_t2part_opts *opts, double *d_sfr,

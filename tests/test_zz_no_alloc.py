@@ -63,3 +63,79 @@ def test_noalloc_rejects_sizes_beyond_limits(dmi):
                                                   C.byref(oo))
     assert rc != 0
     assert b"exceeds" in _lib.load().dmi_last_error(ctx.handle)
+
+
+def _t2part_outputs(torch, opts):
+    X, Y, Z = opts.MatrixSize
+    return {k: torch.full((Z, Y, X), float("nan"), dtype=torch.float64, device="cuda")
+            for k in ("sfr", "sgm", "mfr", "mgm")}
+
+
+def test_noalloc_graph_replays_match_eager_calls(dmi):
+    """dmi_t2map_f64_device_noalloc and dmi_t2part_f64_device recorded into a HIP graph (torch.cuda.graph, the
+    library on the capturing stream through dmi_set_stream): every replay equals the eager calls bit for bit, with
+    the program's own eager work and waits between replays, and a replay after the image changed in place equals
+    eager calls on the new image. The volume spans two selection chunks, so the graph holds two selections and two
+    voxel-kernel launches that read the selected count on the device."""
+    import torch
+    from decaes_amd import device as D
+
+    shape, m, TE, n, reg, K, extra, frac = CALLS[0]
+    print(f"[graph] shape={shape} nTE={m} {reg}", flush=True)
+    img, nin = dmi.synth_mse(shape, m, TE, mask_fraction=frac, seed=37)
+    opts = dmi.T2mapOptions.from_image(img, TE=TE, nT2=n, T2Range=(10e-3, 2.0), Reg=reg, nRefAngles=K,
+                                       SaveRegParam=True, SaveResidualNorm=True, SaveDecayCurve=True, Silent=True)
+    popts = dmi.T2partOptions.from_t2map(opts, SPWin=(10e-3, 40e-3), MPWin=(40e-3, 2.0))
+    dimg = torch.from_numpy(np.ascontiguousarray(img.transpose(3, 2, 1, 0))).to("cuda")
+
+    def eager():
+        out, parts = D.alloc_t2map_outputs(opts, device="cuda"), _t2part_outputs(torch, opts)
+        s = D.t2map_device(dimg, opts, out, noalloc=True)
+        D.t2part_device(out["dist"], popts, parts)
+        torch.cuda.synchronize()
+        return s, {k: v.cpu().numpy() for k, v in {**out, **parts}.items()}
+
+    s_ref, ref = eager()
+    assert s_ref["voxels"] == nin
+    out, parts = D.alloc_t2map_outputs(opts, device="cuda"), _t2part_outputs(torch, opts)
+    g = D.capture(dimg, opts, out, t2part=(popts, parts))
+    other = torch.linspace(0.0, 1.0, 1 << 20, dtype=torch.float64, device="cuda")
+    for r in range(3):
+        print(f"[graph] replay {r}", flush=True)
+        for t in list(out.values()) + list(parts.values()):
+            t.fill_(7.0)
+        other = torch.sin(other) + 0.5
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for k, v in {**out, **parts}.items():
+            assert np.array_equal(v.cpu().numpy(), ref[k], equal_nan=True), (r, k)
+    dimg.mul_(1.5)
+    _, ref2 = eager()
+    assert not np.array_equal(ref2["gdn"], ref["gdn"], equal_nan=True)
+    print("[graph] replay on the changed image", flush=True)
+    g.replay()
+    torch.cuda.synchronize()
+    for k, v in {**out, **parts}.items():
+        assert np.array_equal(v.cpu().numpy(), ref2[k], equal_nan=True), k
+
+
+def test_capture_needs_the_grids_built_outside_capture(dmi):
+    """Under capture, options whose grids are not in the static scratch are refused before anything is recorded."""
+    import torch
+    from decaes_amd import device as D
+
+    print("[graph] refusal under capture", flush=True)
+    img, nin = dmi.synth_mse((4, 4, 2), 32, 10e-3, mask_fraction=1.0, seed=2)
+    opts = dmi.T2mapOptions.from_image(img, TE=10e-3, nT2=36, T2Range=(10e-3, 2.0), Reg="none", nRefAngles=12,
+                                       Silent=True)
+    out = D.alloc_t2map_outputs(opts, device="cuda")
+    dimg = torch.from_numpy(np.ascontiguousarray(img.transpose(3, 2, 1, 0))).to("cuda")
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with pytest.raises(AssertionError, match="outside capture first"):
+        with torch.cuda.graph(g):
+            D.t2map_device(dimg, opts, out, noalloc=True, stream=torch.cuda.current_stream())
+    # the context runs on its own stream again and an eager call works
+    s = D.t2map_device(dimg, opts, out, noalloc=True)
+    assert s["voxels"] == nin
