@@ -71,20 +71,29 @@ def _t2part_outputs(torch, opts):
             for k in ("sfr", "sgm", "mfr", "mgm")}
 
 
-def test_noalloc_graph_replays_match_eager_calls(dmi):
+GRAPH_CASES = {
+    # shape, nTE, TE, nT2, T2Range, Reg, nRefAngles, extra options, mask fraction
+    # two selection chunks: two selections and two voxel-kernel launches that read the selected count on the device
+    "gcv_two_chunks": ((96, 96, 32), 48, 10e-3, 40, (10e-3, 2.0), "gcv", 32, {}, 0.3),
+    # the split schedule: workspace rows beyond the LDS part in the workgroups' global slots
+    "lcurve_split": ((60, 60, 4), 56, 7.9e-3, 60, (8e-3, 2.0), "lcurve", 64, dict(schedule="split"), 0.8),
+}
+
+
+@pytest.mark.parametrize("name", list(GRAPH_CASES))
+def test_noalloc_graph_replays_match_eager_calls(dmi, name):
     """dmi_t2map_f64_device_noalloc and dmi_t2part_f64_device recorded into a HIP graph (torch.cuda.graph, the
     library on the capturing stream through dmi_set_stream): every replay equals the eager calls bit for bit, with
     the program's own eager work and waits between replays, and a replay after the image changed in place equals
-    eager calls on the new image. The volume spans two selection chunks, so the graph holds two selections and two
-    voxel-kernel launches that read the selected count on the device."""
+    eager calls on the new image."""
     import torch
     from decaes_amd import device as D
 
-    shape, m, TE, n, reg, K, extra, frac = CALLS[0]
-    print(f"[graph] shape={shape} nTE={m} {reg}", flush=True)
+    shape, m, TE, n, T2R, reg, K, extra, frac = GRAPH_CASES[name]
+    print(f"[graph] case={name} shape={shape} nTE={m} {reg}", flush=True)
     img, nin = dmi.synth_mse(shape, m, TE, mask_fraction=frac, seed=37)
-    opts = dmi.T2mapOptions.from_image(img, TE=TE, nT2=n, T2Range=(10e-3, 2.0), Reg=reg, nRefAngles=K,
-                                       SaveRegParam=True, SaveResidualNorm=True, SaveDecayCurve=True, Silent=True)
+    opts = dmi.T2mapOptions.from_image(img, TE=TE, nT2=n, T2Range=T2R, Reg=reg, nRefAngles=K, SaveRegParam=True,
+                                       SaveResidualNorm=True, SaveDecayCurve=True, Silent=True, **extra)
     popts = dmi.T2partOptions.from_t2map(opts, SPWin=(10e-3, 40e-3), MPWin=(40e-3, 2.0))
     dimg = torch.from_numpy(np.ascontiguousarray(img.transpose(3, 2, 1, 0))).to("cuda")
 
@@ -97,6 +106,8 @@ def test_noalloc_graph_replays_match_eager_calls(dmi):
 
     s_ref, ref = eager()
     assert s_ref["voxels"] == nin
+    if extra.get("schedule") == "split":
+        assert s_ref["lds_rows"] < m + n
     out, parts = D.alloc_t2map_outputs(opts, device="cuda"), _t2part_outputs(torch, opts)
     g = D.capture(dimg, opts, out, t2part=(popts, parts))
     other = torch.linspace(0.0, 1.0, 1 << 20, dtype=torch.float64, device="cuda")
