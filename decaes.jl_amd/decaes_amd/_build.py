@@ -10,6 +10,8 @@ ROOT = PKG.parent            # decaes.jl_amd/
 REPO = ROOT.parent
 CSRC = ROOT / "csrc"
 LIB = PKG / "libdecaes_amd.so"
+# a C host program of the C ABI (csrc/dmi_run.c), linked against LIB
+RUN = PKG / "dmi_run"
 # Same sources built with -DDMI_PHASE_PROFILE: per-phase shader-clock counters (decaes_amd.phase_profile)
 PHASE_LIB = PKG / "libdecaes_amd_phase.so"
 
@@ -54,6 +56,14 @@ def build(verbose: bool = False, phase_profile: bool = False) -> Path:
         print(" ".join(link), flush=True)
     subprocess.run(link, check=True)
     os.replace(tmp, out)
+    if not phase_profile:
+        run = [os.environ.get("CC", "gcc"), "-O2", "-std=c11", "-Wall", "-I", str(REPO / "include"),
+               str(CSRC / "dmi_run.c"), "-o", str(RUN.with_suffix(".tmp")), "-L", str(PKG), "-ldecaes_amd",
+               "-Wl,-rpath,$ORIGIN", "-Wl,--allow-shlib-undefined", "-lm"]
+        if verbose:
+            print(" ".join(run), flush=True)
+        subprocess.run(run, check=True)
+        os.replace(RUN.with_suffix(".tmp"), RUN)
     return out
 
 
