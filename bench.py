@@ -42,6 +42,10 @@ WORKLOADS = {
     # whole-volume timings compare against
     "c3chi2": dict(shape=(240, 240, 113), nTE=56, TE=7.9e-3, nT2=60, T2Range=(8e-3, 2.0), Reg="chi2", nRefAngles=64,
                    ncomp=4, mask=0.6, Chi2Factor=1.02, slabs=8),
+    # the legacy path (legacy = true: the Dierckx spline flip-angle surrogate over the reference's legacy default of
+    # 8 angles, and the chi2 doubling search) on the 240x240x48 volume with chi2 1.02
+    "c2legacy": dict(shape=(240, 240, 48), nTE=48, TE=10e-3, nT2=40, T2Range=(10e-3, 2.0), Reg="chi2", nRefAngles=8,
+                     ncomp=3, mask=0.6, Chi2Factor=1.02, slabs=1, legacy=True),
 }
 SPWIN = (None, 40e-3)  # short-peak window upper edge (lower edge = T2Range[0])
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
@@ -89,8 +93,10 @@ def cpu_baseline(img, opts_kw, seconds: float):
     oracle_lib.build()
     w = opts_kw
     shape = img.shape[:3]
+    legacy = w.get("legacy", False)
     oo = oracle_lib.make_t2map_opts(shape, w["nTE"], w["TE"], w["nT2"], w["T2Range"], w["Reg"],
-                                    nRefAngles=w["nRefAngles"], Chi2Factor=w["Chi2Factor"])
+                                    nRefAngles=w["nRefAngles"], nRefAnglesMin=w["nRefAngles"] if legacy else None,
+                                    Chi2Factor=w["Chi2Factor"], legacy=legacy)
     # the oracle is FP64 throughout: the baseline is the reference's plain CPU path
     first = img[..., 0].reshape(-1, order="F")
     inmask = np.flatnonzero(first > 0.0)
@@ -369,7 +375,8 @@ def main():
     for a, b in zip(bounds[:-1], bounds[1:]):
         sub = img[:, :, a:b, :]
         kw = dict(MatrixSize=(X, Y, int(b - a)), nTE=w["nTE"], TE=w["TE"], nT2=w["nT2"], T2Range=w["T2Range"],
-                  Reg=w["Reg"], nRefAngles=w["nRefAngles"], Chi2Factor=w["Chi2Factor"], Silent=True)
+                  Reg=w["Reg"], nRefAngles=w["nRefAngles"], Chi2Factor=w["Chi2Factor"], legacy=w.get("legacy", False),
+                  Silent=True)
         o = dmi.T2mapOptions(precision=args.precision, **kw)
         o64 = dmi.T2mapOptions(precision="fp64", **kw)
         po = dmi.T2partOptions.from_t2map(o, SPWin=(w["T2Range"][0], 40e-3), MPWin=(40e-3, w["T2Range"][1]))
@@ -425,6 +432,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {X}x{Y}x{Z} nTE={w['nTE']} TE={w['TE']} nT2={w['nT2']} "
                                f"T2Range={w['T2Range']} Reg={w['Reg']} nRefAngles={w['nRefAngles']} "
+                               f"{'legacy=true ' if w.get('legacy') else ''}"
                                f"mask={w['mask']} precision={args.precision}; one step = one of {nslab} z-slabs "
                                f"of the volume (slices {[sl['z'] for sl in slabs]})",
                    "voxels_processed": nvox, "voxels_in_mask": int(nin), "voxels_volume": X * Y * Z,
@@ -435,7 +443,7 @@ def main():
                                 "chosen_by": "autotuner (measured on the first warmup slab)"}},
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(args.workload, args.precision),
-                     "kernel": f"t2map_kernel<{dmi.api._lib.REG[w['Reg']]}>", "kernel_ms": 1e3 * kernel_s / launches,
+                     "kernel": f"t2map_kernel<{dmi.api._lib.REG[w['Reg']] + (8 if w.get('legacy') else 0)}>", "kernel_ms": 1e3 * kernel_s / launches,
                      "flops_per_launch": flops64 / launches, "fp32_tflops": flops32 / kernel_s / 1e12},
     }
     if args.precision == "mixed" and ran_all and not args.no_plain:
