@@ -309,6 +309,58 @@ def batch(args):
                       "data": "synthetic"}), flush=True)
 
 
+def cli_run(args):
+    """The command line layer end to end on one GPU: the workload's volume is written as a NIfTI file, and
+    decaes_amd.cli runs T2mapSEcorr and T2partSEcorr on it and writes the MAT v7.3 results, as the reference's CLI
+    does (src/main.jl:292-420); each stage's wall time comes from the run's own log."""
+    import tempfile
+    from datetime import datetime
+
+    import decaes_amd as dmi
+    from decaes_amd import cli, fileio
+
+    w = WORKLOADS[args.workload]
+    img, nin = dmi.synth_mse(w["shape"], w["nTE"], w["TE"], ncomp=w["ncomp"], mask_fraction=w["mask"], seed=2024)
+    with tempfile.TemporaryDirectory() as d:
+        src = Path(d) / "volume.nii"
+        fileio.write_nifti(src, img)
+        nbytes_in = src.stat().st_size
+        del img
+        argv = [str(src), "--T2map", "--T2part", "--TE", repr(w["TE"]), "--nT2", str(w["nT2"]), "--T2Range",
+                *map(repr, w["T2Range"]), "--SPWin", repr(w["T2Range"][0]), "40e-3", "--MPWin", "40e-3",
+                repr(w["T2Range"][1]), "--Reg", w["Reg"], "--nRefAngles", str(w["nRefAngles"]), "-o",
+                str(Path(d) / "out"), "--quiet"]
+        if w["Chi2Factor"] is not None:
+            argv += ["--RegParams", repr(w["Chi2Factor"])]
+        if w.get("legacy"):
+            argv += ["--legacy"]
+        t = time.perf_counter()
+        rc = cli.main(argv)
+        wall = time.perf_counter() - t
+        if rc != 0:
+            raise SystemExit(f"the command line failed ({rc})")
+        # stage boundaries from the log's timestamps ("[YYYY-mm-dd HH:MM:SS,mmm] LEVEL message")
+        marks = []
+        for line in (Path(d) / "out" / "volume.log").read_text().splitlines():
+            if line.startswith("["):
+                stamp, msg = line[1:].split("] ", 1)
+                marks.append((datetime.strptime(stamp, "%Y-%m-%d %H:%M:%S,%f"), msg.split(" ", 1)[1]))
+        def at(prefix):
+            return next(ts for ts, m in marks if m.startswith(prefix))
+        stages = {
+            "load_s": (at("Running T2mapSEcorr") - at("Loading input file")).total_seconds(),
+            "t2map_s": (at("Saving T2 distribution") - at("Running T2mapSEcorr")).total_seconds(),
+            "save_t2map_s": (at("Running T2partSEcorr") - at("Saving T2 distribution")).total_seconds(),
+            "t2part_s": (at("Saving T2 parts") - at("Running T2partSEcorr")).total_seconds(),
+            "save_t2parts_s": (at("Finished") - at("Saving T2 parts")).total_seconds(),
+        }
+        out_bytes = sum(f.stat().st_size for f in (Path(d) / "out").iterdir())
+    res = {"cli": {"workload": args.workload, "voxels_in_mask": nin, "input_nifti_bytes": nbytes_in,
+                   "output_mat_bytes": out_bytes, "wall_s": wall, "voxels_per_s_end_to_end": nin / wall,
+                   **stages, "argv": " ".join(argv[1:])}}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -333,6 +385,9 @@ def main():
     ap.add_argument("--window", type=int, default=6)
     ap.add_argument("--compare-alone", action="store_true",
                     help="--batch / --serve: also run each volume or request alone on the device")
+    ap.add_argument("--cli", action="store_true",
+                    help="the command line end to end on the workload's volume written as NIfTI (load, T2map, "
+                         "T2part, MAT v7.3 output)")
     args = ap.parse_args()
     # PyTorch's HIP runtime is loaded before the library, which then binds to it (one runtime per process)
     import torch  # noqa: F401
@@ -349,6 +404,8 @@ def main():
             print("the batched workload runs on one GPU", file=sys.stderr)
             sys.exit(2)
         return batch(args)
+    if args.cli:
+        return cli_run(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); voxels shard across GPUs with no exchange "
               "step, so a multi-GPU figure is N independent single-GPU runs.", file=sys.stderr)
