@@ -109,7 +109,8 @@ int dmi_destroy(dmi_ctx *ctx);
  * NULL. No counterpart in the reference (a Julia task has no stream): it puts this library's kernels in the caller's
  * stream order, and lets a caller capture them into a HIP graph with its own kernels: while `stream` is being
  * captured, dmi_t2map_f64_device_noalloc and dmi_t2part_f64_device record their launches and return at once (see
- * there); the other compute entry points return DMI_EINVAL. ctx never destroys a caller's stream. */
+ * there); the other compute entry points return DMI_EINVAL. ctx never destroys a caller's stream; the stream must
+ * outlive its use by ctx, and the work ctx queued on it must have finished before dmi_destroy(ctx). */
 int dmi_set_stream(dmi_ctx *ctx, void *stream);
 /* Error string of the last failed call on ctx (or of dmi_create when ctx is NULL). */
 const char *dmi_last_error(const dmi_ctx *ctx);
