@@ -121,6 +121,57 @@ def test_b1_map(dmi, oracle):
     assert stable.all(), s
 
 
+COMBOS = {
+    # name: (base case, option overrides, B1 map)
+    "setflip150_lcurve": ("c3_lcurve", dict(SetFlipAngle=150.0), False),
+    "refcon160_gcv": ("c2_gcv", dict(RefConAngle=160.0), False),
+    "minref90_t1_1p5_chi2": ("c1_chi2", dict(MinRefAngle=90.0, T1=1.5), False),
+    "threshold_q40_chi2": ("c1_chi2", dict(Threshold="q40"), False),
+    "k16_kmin3_mdp": ("mdp", dict(nRefAngles=16, nRefAnglesMin=3), False),
+    "b1_lcurve": ("c3_lcurve", {}, True),
+}
+
+
+@pytest.mark.parametrize("name", list(COMBOS))
+def test_option_combinations_match_oracle(dmi, oracle, name):
+    """Options in combinations the configurations do not cover, every voxel against the oracle: a fixed flip angle
+    with the L-curve, RefConAngle != 180 with GCV, another MinRefAngle and T1, a positive threshold, a short angle
+    grid with a small nRefAnglesMin, and a B1 map with the L-curve. The oracle's options are taken field by field
+    from the same T2mapOptions."""
+    print(f"[parity] combo={name}", flush=True)
+    base, over, b1 = COMBOS[name]
+    shape, m, TE, n, T2R, reg, K, extra, ncomp, frac = CASES[base]
+    img, _ = dmi.synth_mse(shape, m, TE, ncomp=ncomp, mask_fraction=frac, seed=29)
+    over = dict(over)
+    if over.get("Threshold") == "q40":
+        e1 = img[..., 0]
+        over["Threshold"] = float(np.quantile(e1[e1 > 0], 0.4))
+    kw = dict(TE=TE, nT2=n, T2Range=T2R, Reg=reg, nRefAngles=K, SaveRegParam=True, SaveResidualNorm=True,
+              SaveDecayCurve=True, Silent=True, **extra)
+    kw.update(over)
+    opts = dmi.T2mapOptions.from_image(img, **kw)
+    alpha = np.asfortranarray(np.random.default_rng(4).uniform(130, 180, shape)) if b1 else None
+    maps, dist = dmi.T2mapSEcorr(img, opts, alpha=alpha)
+    oo = oracle.make_t2map_opts(shape, m, TE, n, T2R, reg, T1=opts.T1, Threshold=opts.Threshold,
+                                MinRefAngle=opts.MinRefAngle, nRefAngles=opts.nRefAngles,
+                                nRefAnglesMin=opts.nRefAnglesMin, Chi2Factor=opts.Chi2Factor,
+                                NoiseLevel=opts.NoiseLevel, RefConAngle=opts.RefConAngle,
+                                SetFlipAngle=opts.SetFlipAngle)
+    omaps, odist, sel, spread = PU.oracle_ensemble(oracle, img, oo, T2R, alpha_in=alpha)
+    assert np.array_equal(np.isnan(maps["gdn"]), ~sel) and sel.sum() > 0
+    if "Threshold" in over:
+        e1 = img[..., 0]
+        assert np.array_equal(sel, e1 > over["Threshold"]) and 0 < sel.sum() < (e1 > 0).sum()
+    if b1:
+        assert np.array_equal(maps["alpha"], alpha)
+    if opts.SetFlipAngle is not None:
+        assert np.all(maps["alpha"][sel] == opts.SetFlipAngle)
+    s, stable = PU.check(name, oracle, T2R, sel, spread, maps, dist, omaps, odist,
+                         extra_check=_curves_check(maps, omaps, sel))
+    if reg in ("chi2", "mdp", "none"):
+        assert stable.all(), s
+
+
 def test_t2part_matches_oracle(dmi, oracle):
     rng = np.random.default_rng(3)
     dist = np.asfortranarray(rng.uniform(0, 1, (9, 7, 3, 40)) * (rng.uniform(0, 1, (9, 7, 3, 40)) > 0.5))
