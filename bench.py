@@ -199,7 +199,7 @@ def serve_levels(args) -> list:
             reqs.append((sub, o, po))
         gaps = rng.exponential(1.0 / qps, size=len(reqs))
         lat = []
-        with Server() as srv:
+        with Server(order=args.order) as srv:
             handles = []
             t0 = time.perf_counter()
             waiter_done = threading.Event()
@@ -229,7 +229,8 @@ def serve_levels(args) -> list:
             st = srv.stats()
         levels.append({"qps": qps, "requests": len(reqs), "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
                        "voxels": st["voxels"], "voxels_per_s_busy": st["voxels"] / (st["busy_ms"] * 1e-3),
-                       "voxels_per_s_wall": st["voxels"] / wall, "failed": st["failed"]})
+                       "voxels_per_s_wall": st["voxels"] / wall, "failed": st["failed"],
+                       "order": getattr(args, "order", "shortest")})
         if getattr(args, "compare_alone", False):
             a = alone_rate(reqs)
             a["served_busy_over_alone"] = levels[-1]["voxels_per_s_busy"] / a["device_voxels_per_s"]
@@ -380,6 +381,8 @@ def main():
     ap.add_argument("--requests", type=int, default=16)
     ap.add_argument("--max-slices", type=int, default=8)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--order", default="shortest", choices=["shortest", "turns"],
+                    help="served stream: which request of a stream gets the next chunk")
     ap.add_argument("--batch", action="store_true")
     ap.add_argument("--volumes", type=int, default=64)
     ap.add_argument("--window", type=int, default=6)
@@ -549,10 +552,11 @@ def main():
         # capacity, so the percentiles are service latencies rather than backlog drain times
         log("[bench] served stream (12 requests of 1-4 slices of 240x240 at QPS 2)")
         try:
-            lv = serve_levels(argparse.Namespace(qps="2", requests=12, max_slices=4, seed=7))[0]
+            lv = serve_levels(argparse.Namespace(qps="2", requests=12, max_slices=4, seed=7, order=args.order))[0]
             res["served"] = {"qps": lv["qps"], "requests": lv["requests"], "p50_ms": lv["p50_ms"],
                              "p99_ms": lv["p99_ms"], "voxels_per_s_busy": lv["voxels_per_s_busy"],
-                             "failed": lv["failed"], "stream": "1-4 slices of 240x240, nTE 32/48/56, chi2/gcv/lcurve"}
+                             "failed": lv["failed"], "order": lv["order"],
+                             "stream": "1-4 slices of 240x240, nTE 32/48/56, chi2/gcv/lcurve"}
         except Exception as e:  # the throughput line is still printed
             res["served"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu_baseline:

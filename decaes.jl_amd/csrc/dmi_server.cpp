@@ -2,8 +2,8 @@
 //
 // Two host threads drive the device. The scheduler admits queued requests in arrival order while their device
 // footprint fits the HBM budget, binds each to a stream whose cached grid bases match its options (or rebinds an
-// idle stream), selects its voxels, and keeps up to two voxel chunks in flight per stream, taking chunks from the
-// stream's requests in turn. The copy thread moves host arrays in and out (pageable copies block, so they stay
+// idle stream), selects its voxels, and keeps up to two voxel chunks in flight per stream, each from the stream's
+// request with the fewest voxels left to launch (or from its requests in turn). The copy thread moves host arrays in and out (pageable copies block, so they stay
 // off the scheduler) and frees a request's device memory once its outputs are home. Request memory is
 // stream-ordered (hipMallocAsync / hipFreeAsync on the copy stream), so admission never waits on running kernels.
 #include <hip/hip_runtime.h>
@@ -462,16 +462,24 @@ void sched_loop(dmi_server *S) {
                 }
                 progress = true;
             }
-            // keep up to kMaxInflight chunks queued, taking the stream's requests in turn
+            // keep up to kMaxInflight chunks queued: the stream's request with the fewest voxels left to launch, or
+            // its requests in turn (dmi_server_config.order)
             while ((int)st.inflight.size() < kMaxInflight && !st.members.empty()) {
                 Request *r = nullptr;
-                for (size_t t = 0; t < st.members.size(); ++t) {
-                    Request *c = st.members[(st.rr + t) % st.members.size()];
-                    if (c->st == St::Ready && c->launched < c->nvox) {
-                        r = c;
-                        st.rr = (st.rr + t + 1) % st.members.size();
-                        break;
+                if (S->cfg.order == DMI_ORDER_TURNS) {
+                    for (size_t t = 0; t < st.members.size(); ++t) {
+                        Request *c = st.members[(st.rr + t) % st.members.size()];
+                        if (c->st == St::Ready && c->launched < c->nvox) {
+                            r = c;
+                            st.rr = (st.rr + t + 1) % st.members.size();
+                            break;
+                        }
                     }
+                } else {
+                    for (Request *c : st.members)
+                        if (c->st == St::Ready && c->launched < c->nvox &&
+                            (!r || c->nvox - c->launched < r->nvox - r->launched))
+                            r = c;
                 }
                 if (!r) break;
                 if (r->tune) {
@@ -562,6 +570,7 @@ int dmi_server_create(dmi_server **out, int device, const dmi_server_config *cfg
     if (cfg) S->cfg = *cfg;
     if (S->cfg.max_streams <= 0) S->cfg.max_streams = 4;
     if (S->cfg.chunk_voxels <= 0) S->cfg.chunk_voxels = 65536;
+    if (S->cfg.order != DMI_ORDER_TURNS) S->cfg.order = DMI_ORDER_SHORTEST;
     if (S->cfg.hbm_budget_bytes <= 0) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) return DMI_EHIP;

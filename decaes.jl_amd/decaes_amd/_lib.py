@@ -75,7 +75,8 @@ class StatsC(C.Structure):
 
 
 class ServerConfigC(C.Structure):
-    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_streams", C.c_int32), ("chunk_voxels", C.c_int32)]
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_streams", C.c_int32), ("chunk_voxels", C.c_int32),
+                ("order", C.c_int32), ("reserved", C.c_int32)]
 
 
 class RequestInfoC(C.Structure):

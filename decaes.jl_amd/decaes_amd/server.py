@@ -36,9 +36,15 @@ class Request:
 
 
 class Server:
-    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, max_streams: int = 0, chunk_voxels: int = 0):
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, max_streams: int = 0, chunk_voxels: int = 0,
+                 order: str = "shortest"):
+        """order: "shortest" (the request with the fewest voxels left gets a stream's next chunk) or "turns" (a
+        stream's requests take turns, one chunk each); dmi_server_config.order."""
+        if order not in ("shortest", "turns"):
+            raise ValueError(f"order must be 'shortest' or 'turns', not {order!r}")
         L = _lib.load()
-        cfg = _lib.ServerConfigC(int(hbm_budget_bytes), int(max_streams), int(chunk_voxels))
+        cfg = _lib.ServerConfigC(int(hbm_budget_bytes), int(max_streams), int(chunk_voxels),
+                                 0 if order == "shortest" else 1, 0)
         h = C.c_void_p()
         rc = L.dmi_server_create(C.byref(h), int(device), C.byref(cfg))
         if rc != _lib.DMI_OK:

@@ -175,7 +175,8 @@ int dmi_last_stats(const dmi_ctx *ctx, dmi_stats *out);
  * process alive and runs one CLI call at a time). Requests (a T2mapSEcorr call, optionally followed by
  * T2partSEcorr on its distribution) are admitted while their device footprint fits the HBM budget, split into
  * voxel chunks, and interleaved on the GPU: requests with the same grid options share a stream and its cached
- * grid bases and take turns chunk by chunk; different option sets run on separate streams concurrently. A request
+ * grid bases, whose next chunk goes to the request with the fewest voxels left (or to each in turn,
+ * dmi_server_config.order); different option sets run on separate streams concurrently. A request
  * that arrives while others run starts at the next chunk boundary. Each voxel's result is the same as in a
  * dmi_t2map_f64 call (voxels are independent and the per-voxel computation does not depend on the batch).
  * Host arrays passed to dmi_server_submit must stay valid until dmi_server_wait returns for that request. */
@@ -185,7 +186,13 @@ typedef struct {
     int64_t hbm_budget_bytes; /* device memory for admitted requests; <= 0 selects half of the free memory */
     int32_t max_streams;      /* option sets in flight at once, one HIP stream each; <= 0 selects 4 */
     int32_t chunk_voxels;     /* voxels per launch; <= 0 selects 65536 */
+    int32_t order;            /* which admitted request of a stream gets the next chunk: DMI_ORDER_* */
+    int32_t reserved;         /* pass 0 */
 } dmi_server_config;
+
+/* dmi_server_config.order. SHORTEST (the default, 0): the request with the fewest voxels left to launch, the earliest
+ * admitted on a tie, so short requests overtake long ones. TURNS: the stream's requests take turns, one chunk each. */
+enum { DMI_ORDER_SHORTEST = 0, DMI_ORDER_TURNS = 1 };
 
 typedef struct {
     int32_t status;           /* DMI_OK or the request's error code (dmi_server_request_error has the text) */
