@@ -84,3 +84,17 @@ def test_context_without_device_fails_loudly(dmi):
     with pytest.raises(dmi.DmiError) as e:
         dmi.Context(0)
     assert e.value.code == -5
+
+
+def test_server_config_layout_and_order_check():
+    """dmi_server_config as declared: int64 budget, then four int32 (streams, chunk, order, reserved)."""
+    import ctypes as C
+
+    from decaes_amd import _lib
+    from decaes_amd.server import Server
+
+    assert C.sizeof(_lib.ServerConfigC) == 24
+    assert [f[0] for f in _lib.ServerConfigC._fields_] == ["hbm_budget_bytes", "max_streams", "chunk_voxels", "order",
+                                                           "reserved"]
+    with pytest.raises(ValueError, match="order"):
+        Server(order="fifo")

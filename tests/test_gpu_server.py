@@ -35,12 +35,13 @@ def _same(a, b):
     np.testing.assert_array_equal(a, b)  # NaN == NaN here
 
 
-def test_server_matches_alone_in_the_mix(dmi):
+@pytest.mark.parametrize("order", ["shortest", "turns"])
+def test_server_matches_alone_in_the_mix(dmi, order):
     from decaes_amd.server import Server
     cases = _cases(dmi)
     ref = [_alone(dmi, *c) for c in cases]
-    print("[server] mix of", len(cases), "requests, chunk 37 voxels, 2 streams", flush=True)
-    with Server(max_streams=2, chunk_voxels=37) as srv:
+    print("[server] mix of", len(cases), "requests, chunk 37 voxels, 2 streams, order", order, flush=True)
+    with Server(max_streams=2, chunk_voxels=37, order=order) as srv:
         hs = [srv.submit(*c[:2], part_opts=c[2]) for c in cases]
         for h, (rmaps, rdist, rparts) in zip(hs, ref):
             maps, dist, parts, info = srv.wait(h)
