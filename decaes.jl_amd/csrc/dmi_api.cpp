@@ -37,6 +37,9 @@ struct dmi_ctx {
     DevBuf f32;
     DevBuf weights, tlog;
     std::vector<double> tlog_host, weights_host;  // the T2part tables last uploaded to tlog / weights
+    // dmi_t2map_f64 on volumes of several chunks: a copy stream and two staging sets (image, B1 map, outputs)
+    hipStream_t copy_stream = nullptr;
+    DevBuf pimg[2], pain[2], pout[2][12], pcnt;
 };
 
 namespace {
@@ -240,6 +243,196 @@ int run_t2map(dmi_ctx *c, const double *d_image, const dmi_t2map_opts *o, const 
     return run_t2map_on(c, d_image, o, d_alpha_in, d, rb);
 }
 
+// dmi_t2map_f64 on a volume of more than one chunk. The voxels are taken as consecutive ranges of kHostChunk linear
+// indices, each handled as a volume of its own (MatrixSize (len, 1, 1): a voxel's result does not depend on the
+// volume around it), in one of two device staging sets. The copy stream moves the next chunk's image rows and the
+// outputs' current contents in, and the previous chunk's outputs out, while ctx's stream runs the voxel kernel on
+// the current chunk; each chunk's count of selected voxels stays on the device, so no chunk waits for the host.
+// hp[i] / rows[i]: the caller's per-voxel output arrays and their rows of V doubles (NULL: not requested).
+constexpr int64_t kHostChunk = 524288;
+
+struct PipeEvents {
+    hipEvent_t in[2] = {nullptr, nullptr}, done[2] = {nullptr, nullptr};
+    ~PipeEvents() {
+        for (int b = 0; b < 2; ++b) {
+            if (in[b]) (void)hipEventDestroy(in[b]);
+            if (done[b]) (void)hipEventDestroy(done[b]);
+        }
+    }
+};
+
+int run_t2map_host_chunks(dmi_ctx *c, const double *image, const dmi_t2map_opts *o, const double *alpha_in,
+                          double *const hp[12], const int64_t rows[12]) {
+    const int m = o->nTE, n = o->nT2;
+    const int64_t V = o->MatrixSize[0] * o->MatrixSize[1] * o->MatrixSize[2];
+    const int64_t C = kHostChunk, nch = (V + C - 1) / C;
+    const bool fixed = !std::isnan(o->SetFlipAngle);
+    HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+    int rc = c->grids.ensure(c->err, o, c->stream);
+    if (rc) return rc;
+    if (!c->copy_stream) HIPCHECK(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    hipStream_t cs = c->copy_stream;
+    HIPCHECK(c, c->vox.ensure(sizeof(int64_t) * C));
+    HIPCHECK(c, c->lb.counter.ensure(sizeof(unsigned) * 4));
+    HIPCHECK(c, c->lb.counters.ensure(sizeof(unsigned long long) * 4));
+    HIPCHECK(c, c->lb.status.ensure(sizeof(int) * 4));
+    HIPCHECK(c, c->pcnt.ensure(sizeof(unsigned) * (size_t)nch));
+    // each chunk as a volume of its own; the fixed basis of SetFlipAngle is copied once at the end
+    dmi_t2map_opts oc = *o;
+    oc.MatrixSize[0] = C;
+    oc.MatrixSize[1] = 1;
+    oc.MatrixSize[2] = 1;
+    if (fixed) oc.SaveNNLSBasis = 0;
+    std::vector<dmi_host::Schedule> cands;
+    rc = dmi_host::t2map_schedules(c->err, &oc, cands);
+    if (rc) return rc;
+    int64_t slot_doubles = 0;
+    for (const auto &s : cands)
+        slot_doubles = std::max<int64_t>(slot_doubles, dmi::t2map_slot_doubles(m, n, s.rl >= m + n ? 0 : s.gld) *
+                                                           std::max<int64_t>(1, std::min<int64_t>((int64_t)c->num_cu * std::max(s.per_cu, 1), C)));
+    HIPCHECK(c, c->lb.scratch.ensure(sizeof(double) * (size_t)slot_doubles));
+    for (int b = 0; b < 2; ++b) {
+        HIPCHECK(c, c->pimg[b].ensure(sizeof(double) * (size_t)(C * m)));
+        if (alpha_in) HIPCHECK(c, c->pain[b].ensure(sizeof(double) * (size_t)C));
+        for (int i = 0; i < 12; ++i)
+            if (hp[i]) HIPCHECK(c, c->pout[b][i].ensure(sizeof(double) * (size_t)(C * rows[i])));
+    }
+    RunBufs rb{c->grids.ptrs(), c->vox.as<int64_t>(), C, c->lb.counter.as<unsigned>(),
+               c->lb.counters.as<unsigned long long>(), c->lb.status.as<int>(), c->lb.scratch.as<double>(), slot_doubles};
+    unsigned *cnt = c->pcnt.as<unsigned>();
+    auto span = [&](int64_t k, int64_t &v0, int64_t &len) { v0 = k * C; len = std::min<int64_t>(C, V - v0); };
+    // chunk k's rows between the caller's arrays (row pitch V) and staging set b (row pitch len)
+    auto stage = [&](int64_t k, int b, bool in) -> int {
+        int64_t v0, len;
+        span(k, v0, len);
+        const size_t w = sizeof(double) * (size_t)len, hpitch = sizeof(double) * (size_t)V;
+        if (in) {
+            HIPCHECK(c, hipMemcpy2DAsync(c->pimg[b].p, w, image + v0, hpitch, w, m, hipMemcpyHostToDevice, cs));
+            if (alpha_in) HIPCHECK(c, hipMemcpyAsync(c->pain[b].p, alpha_in + v0, w, hipMemcpyHostToDevice, cs));
+        }
+        for (int i = 0; i < 12; ++i) {
+            if (!hp[i]) continue;
+            if (in)
+                HIPCHECK(c, hipMemcpy2DAsync(c->pout[b][i].p, w, hp[i] + v0, hpitch, w, (size_t)rows[i],
+                                             hipMemcpyHostToDevice, cs));
+            else
+                HIPCHECK(c, hipMemcpy2DAsync(hp[i] + v0, hpitch, c->pout[b][i].p, w, w, (size_t)rows[i],
+                                             hipMemcpyDeviceToHost, cs));
+        }
+        return DMI_OK;
+    };
+    auto chunk_opts = [&](int64_t k) {
+        int64_t v0, len;
+        span(k, v0, len);
+        dmi_t2map_opts ok = oc;
+        ok.MatrixSize[0] = len;
+        return ok;
+    };
+    auto chunk_out = [&](int b) {
+        double *d[12];
+        for (int i = 0; i < 12; ++i) d[i] = hp[i] ? c->pout[b][i].as<double>() : nullptr;
+        return dmi_t2map_out{d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11]};
+    };
+    c->stats = dmi_stats{};
+    dmi_host::Schedule sched;
+    int64_t k0 = 0, voxels = 0;
+    double kms0 = 0.0;
+    const bool tune = oc.schedule == DMI_SCHED_AUTO && cands.size() > 1 && !dmi_host::schedule_lookup(&oc, sched);
+    if (tune) {
+        // the first chunk on its own, with the autotuner, which waits for its trial launches
+        if ((rc = stage(0, 0, true))) return rc;
+        HIPCHECK(c, hipStreamSynchronize(cs));
+        const dmi_t2map_opts o0 = chunk_opts(0);
+        const dmi_t2map_out d0 = chunk_out(0);
+        rc = run_t2map_on(c, c->pimg[0].as<double>(), &o0, alpha_in ? c->pain[0].as<double>() : nullptr, &d0, rb);
+        if (rc) return rc;
+        if ((rc = stage(0, 0, false))) return rc;
+        HIPCHECK(c, hipStreamSynchronize(cs));
+        voxels = c->stats.voxels;
+        kms0 = c->stats.kernel_ms;
+        k0 = 1;
+    } else {
+        HIPCHECK(c, dmi::launch_fill(rb.counters, sizeof(unsigned long long) * 4, 0u, c->stream));
+        HIPCHECK(c, dmi::launch_fill(rb.status, sizeof(int) * 4, 0u, c->stream));
+    }
+    rc = dmi_host::t2map_schedule(c->err, &oc, sched);
+    if (rc) return rc;
+    PipeEvents ev;
+    for (int b = 0; b < 2; ++b) {
+        HIPCHECK(c, hipEventCreateWithFlags(&ev.in[b], hipEventDisableTiming));
+        HIPCHECK(c, hipEventCreateWithFlags(&ev.done[b], hipEventDisableTiming));
+    }
+    auto pipeline = [&]() -> int {
+        if (k0 >= nch) return DMI_OK;
+        HIPCHECK(c, dmi::launch_fill(cnt + k0, sizeof(unsigned) * (size_t)(nch - k0), 0u, c->stream));
+        // the copy stream starts after the counts are cleared and the grids are in place
+        HIPCHECK(c, hipEventRecord(ev.done[1], c->stream));
+        HIPCHECK(c, hipStreamWaitEvent(cs, ev.done[1], 0));
+        int r = stage(k0, (int)(k0 % 2), true);
+        if (r) return r;
+        HIPCHECK(c, hipEventRecord(ev.in[k0 % 2], cs));
+        HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
+        for (int64_t k = k0; k < nch; ++k) {
+            const int b = (int)(k % 2), b1 = 1 - b;
+            HIPCHECK(c, hipStreamWaitEvent(c->stream, ev.in[b], 0));
+            const dmi_t2map_opts ok = chunk_opts(k);
+            const dmi_t2map_out dk = chunk_out(b);
+            const int64_t len = ok.MatrixSize[0];
+            dmi::T2mapParams p = dmi_host::t2map_params(&ok, rb.g, c->pimg[b].as<double>(),
+                                                        alpha_in ? c->pain[b].as<double>() : nullptr, &dk);
+            dmi_host::apply_schedule(p, sched);
+            p.nvox_dev = cnt + k;
+            HIPCHECK(c, dmi::launch_select(c->pimg[b].as<double>(), 0, len, o->Threshold, rb.vox, cnt + k, c->stream));
+            r = dmi_host::t2map_launch_raw(c->err, p, rb.vox, len, rb.counter + 1, rb.counters, rb.status, rb.slots,
+                                           rb.slot_doubles, c->num_cu, c->stream);
+            if (r) return r;
+            HIPCHECK(c, hipEventRecord(ev.done[b], c->stream));
+            if (k + 1 < nch) {
+                // staging set b1 held chunk k - 1: its outputs go home before chunk k + 1 moves in
+                if (k - 1 >= k0) {
+                    HIPCHECK(c, hipStreamWaitEvent(cs, ev.done[b1], 0));
+                    if ((r = stage(k - 1, b1, false))) return r;
+                }
+                if ((r = stage(k + 1, b1, true))) return r;
+                HIPCHECK(c, hipEventRecord(ev.in[b1], cs));
+            }
+        }
+        HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
+        for (int64_t k = std::max(k0, nch - 2); k < nch; ++k) {
+            HIPCHECK(c, hipStreamWaitEvent(cs, ev.done[k % 2], 0));
+            if ((r = stage(k, (int)(k % 2), false))) return r;
+        }
+        return DMI_OK;
+    };
+    // on a failure both streams drain first, so no queued copy or kernel outlives the call
+    if ((rc = pipeline())) {
+        (void)hipStreamSynchronize(cs);
+        (void)hipStreamSynchronize(c->stream);
+        return rc;
+    }
+    HIPCHECK(c, hipStreamSynchronize(cs));
+    HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
+    HIPCHECK(c, hipEventSynchronize(c->ev[3]));
+    std::vector<unsigned> counts((size_t)nch, 0u);
+    unsigned long long tot[4] = {0, 0, 0, 0};
+    HIPCHECK(c, hipMemcpy(counts.data(), cnt, sizeof(unsigned) * (size_t)nch, hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(tot, rb.counters, sizeof tot, hipMemcpyDeviceToHost));
+    for (int64_t k = k0; k < nch; ++k) voxels += std::min<int64_t>(counts[(size_t)k], std::min<int64_t>(C, V - k * C));
+    float kms = 0, tms = 0;
+    if (k0 < nch) HIPCHECK(c, hipEventElapsedTime(&kms, c->ev[1], c->ev[2]));
+    HIPCHECK(c, hipEventElapsedTime(&tms, c->ev[0], c->ev[3]));
+    c->stats.voxels = voxels;
+    c->stats.fp64_flops = (int64_t)tot[0];
+    c->stats.nnls_solves = (int64_t)tot[1];
+    c->stats.fp32_flops = (int64_t)tot[2];
+    c->stats.voxels_failed = (int64_t)tot[3];
+    c->stats.kernel_ms = kms0 + kms;
+    c->stats.total_ms = tms;
+    c->stats.lds_rows = std::min(sched.rl, m + n);
+    c->stats.workgroups_per_cu = sched.per_cu;
+    return DMI_OK;
+}
+
 // The allocation-free form: every buffer lives in the device's static scratch. The grids in it are keyed per
 // device (they belong to whichever call last built them there).
 std::mutex g_noalloc_mu;
@@ -378,6 +571,16 @@ int dmi_destroy(dmi_ctx *c) {
     for (auto *b : bufs) b->release();
     for (auto &b : c->out) b.release();
     for (auto &b : c->t2p) b.release();
+    for (int k = 0; k < 2; ++k) {
+        c->pimg[k].release();
+        c->pain[k].release();
+        for (auto &b : c->pout[k]) b.release();
+    }
+    c->pcnt.release();
+    if (c->copy_stream) {
+        (void)hipStreamSynchronize(c->copy_stream);
+        (void)hipStreamDestroy(c->copy_stream);
+    }
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -443,6 +646,20 @@ int dmi_t2map_f64(dmi_ctx *c, const double *image, const dmi_t2map_opts *o, cons
     const int m = o->nTE, n = o->nT2;
     const int64_t V = o->MatrixSize[0] * o->MatrixSize[1] * o->MatrixSize[2];
     const bool fixed = !std::isnan(o->SetFlipAngle);
+    if (V > kHostChunk && !(o->SaveNNLSBasis && !fixed)) {
+        // copies overlapped with the voxel kernel, chunk by chunk; the per-voxel bases of SaveNNLSBasis take the
+        // one-piece path below
+        double *hp[12] = {out->dist, out->gdn, out->ggm, out->gva, out->fnr, out->snr, out->alpha,
+                          o->SaveResidualNorm ? out->resnorm : nullptr, o->SaveDecayCurve ? out->decaycurve : nullptr,
+                          o->SaveRegParam ? out->mu : nullptr, o->SaveRegParam ? out->chi2factor : nullptr, nullptr};
+        const int64_t rows[12] = {n, 1, 1, 1, 1, 1, 1, 1, m, 1, 1, 0};
+        rc = run_t2map_host_chunks(c, image, o, alpha_in, hp, rows);
+        if (rc) return rc;
+        if (fixed && o->SaveNNLSBasis)  // the shared nTE x nT2 basis, from its column-major copy in the grids
+            HIPCHECK(c, hipMemcpy2D(out->decaybasis, sizeof(double) * m, c->grids.ptrs().AsT,
+                                    sizeof(double) * dmi::basis_ld(m), sizeof(double) * m, n, hipMemcpyDeviceToHost));
+        return DMI_OK;
+    }
     HIPCHECK(c, c->img.ensure(sizeof(double) * (size_t)V * m));
     HIPCHECK(c, hipMemcpyAsync(c->img.p, image, sizeof(double) * (size_t)V * m, hipMemcpyHostToDevice, c->stream));
     if (alpha_in) {
