@@ -267,7 +267,7 @@ int t2map_launch_raw(std::string &err, dmi::T2mapParams p, const int64_t *d_vox,
     if (nvox <= 0) return DMI_OK;
     const size_t lds = dmi::t2map_lds_bytes(p.m, p.n, p.lda, p.K, p.legacy);
     int per_cu = 0;
-    DMI_HIP(err, dmi::t2map_occupancy(p.reg, p.legacy, &per_cu, lds));
+    DMI_HIP(err, dmi::t2map_occupancy(p.reg, p.legacy, &per_cu, lds, p.rl < p.mw));
     if (per_cu < 1) return errorf(err, DMI_EUNSUPPORTED, "voxel kernel does not fit on a CU (LDS %zu bytes)", lds);
     p.slot_stride = dmi::t2map_slot_doubles(p.m, p.n, p.gld);
     const int64_t fit = slot_doubles / p.slot_stride;
@@ -293,7 +293,7 @@ int t2map_launch(std::string &err, dmi::T2mapParams p, const int64_t *d_vox, int
     DMI_HIP(err, lb.status.ensure(sizeof(int) * 4));
     const size_t lds = dmi::t2map_lds_bytes(p.m, p.n, p.lda, p.K, p.legacy);
     int per_cu = 0;
-    DMI_HIP(err, dmi::t2map_occupancy(p.reg, p.legacy, &per_cu, lds));
+    DMI_HIP(err, dmi::t2map_occupancy(p.reg, p.legacy, &per_cu, lds, p.rl < p.mw));
     if (per_cu < 1) return errorf(err, DMI_EUNSUPPORTED, "voxel kernel does not fit on a CU (LDS %zu bytes)", lds);
     const int64_t grid = std::min<int64_t>((int64_t)num_cu * per_cu, nvox);
     const int64_t slot_doubles = dmi::t2map_slot_doubles(p.m, p.n, p.gld) * grid;
@@ -320,7 +320,7 @@ int t2map_schedules(std::string &err, const dmi_t2map_opts *o, std::vector<Sched
     for (int target = full.per_cu + 1; target <= full.per_cu + 2; ++target) {
         for (int lda = full.lda - 4; lda >= rmin; lda -= 4) {
             int pc = 0;
-            DMI_HIP(err, dmi::t2map_occupancy(o->Reg, o->legacy, &pc, dmi::t2map_lds_bytes(m, n, lda, K, o->legacy)));
+            DMI_HIP(err, dmi::t2map_occupancy(o->Reg, o->legacy, &pc, dmi::t2map_lds_bytes(m, n, lda, K, o->legacy), true));
             if (pc >= target) {
                 if (pc > best) {
                     Schedule s;

@@ -116,6 +116,10 @@ hipError_t launch_t2part(const T2partParams &p, hipStream_t s);
 // Float32 <-> Float64 element-wise conversion (Float32 entry points)
 hipError_t launch_widen(const float *src, double *dst, int64_t n, hipStream_t s);
 hipError_t launch_narrow(const double *src, float *dst, int64_t n, hipStream_t s);
-hipError_t t2map_occupancy(int reg, int legacy, int *blocks_per_cu, size_t lds);
+// split: of the voxel kernel built for the split schedule (rows beyond an LDS cutoff in the global slot)
+hipError_t t2map_occupancy(int reg, int legacy, int *blocks_per_cu, size_t lds, bool split = false);
+// the voxel kernel of the split schedule, from dmi_kernels_split.hip (launch_t2map calls it when p.rl < p.mw)
+hipError_t launch_t2map_split(const T2mapParams &p, int grid, size_t lds, hipStream_t s);
+hipError_t t2map_occupancy_split(int reg, int legacy, int *blocks_per_cu, size_t lds);
 
 }  // namespace dmi

@@ -12,6 +12,19 @@
 //
 // Arithmetic follows the reference statement by statement, with fma() where DECAES writes muladd or sits
 // under @muladd (src/NNLS.jl:47, src/EPGdecaycurve.jl:858-1024); built with -ffp-contract=off.
+//
+// This file is compiled twice. As itself it builds every kernel, the voxel kernel with every workspace row in LDS
+// (the slot branches of the split schedule compile away) and the host launchers; dmi_kernels_split.hip includes it
+// with DMI_SPLIT_TU = 1 to build only the voxel kernel for the split schedule, whose rows beyond an LDS cutoff live in
+// the workgroup's global slot. The instrumented build (DMI_PHASE_PROFILE) is one unit with the split branches kept.
+#ifndef DMI_SPLIT_TU
+#define DMI_SPLIT_TU 0
+#endif
+#if DMI_SPLIT_TU || defined(DMI_PHASE_PROFILE)
+#define DMI_SLOT_ROWS 1
+#else
+#define DMI_SLOT_ROWS 0
+#endif
 #include <hip/hip_runtime.h>
 #include <mutex>
 #include <type_traits>
@@ -534,7 +547,9 @@ __global__ void fill_kernel(unsigned int *p, int64_t nwords, unsigned int v) {
 // each device holds for the life of the process. Host code never passes `scratch` itself to a kernel: it takes
 // the device's own address of the array once with hipGetSymbolAddress (noalloc_scratch below).
 // ------------------------------------------------------------------------------------------------
+#if !DMI_SPLIT_TU
 static __device__ __align__(256) double scratch[kNoAllocDoubles];
+#endif
 
 // ------------------------------------------------------------------------------------------------
 // Per-voxel wave state
@@ -557,6 +572,8 @@ struct Wave {
     // CU's LDS). rl is even and at least max(basis_ld(m), n + 2): the basis rows and every row of the triangular
     // factor and of a reflector's head stay in LDS; only Tikhonov rows appended late live in G.
     int rl, gld;
+    // Every test of a row against rl below is written `!DMI_SLOT_ROWS || ...` (or `DMI_SLOT_ROWS && ...`), so the
+    // unit built without slot rows keeps no slot code at all.
     gdbl *G;
     // the dual w_j of position j = L (the reference's w), in a register: only this lane writes or reads it
     double wl;
@@ -570,9 +587,11 @@ struct Wave {
 
     __device__ __forceinline__ ldbl &Aat(int i, int j) { return A[j * lda + i]; }
     // row i of column j wherever it lives (i < rl: LDS, else the global slot); per-lane rows may differ
-    __device__ __forceinline__ double Aget(int i, int j) { return (i < rl) ? (double)A[j * lda + i] : (double)G[j * gld + (i - rl)]; }
+    __device__ __forceinline__ double Aget(int i, int j) {
+        return (!DMI_SLOT_ROWS || i < rl) ? (double)A[j * lda + i] : (double)G[j * gld + (i - rl)];
+    }
     __device__ __forceinline__ void Aput(int i, int j, double v) {
-        if (i < rl) A[j * lda + i] = v;
+        if (!DMI_SLOT_ROWS || i < rl) A[j * lda + i] = v;
         else G[j * gld + (i - rl)] = v;
     }
 
@@ -666,7 +685,7 @@ struct Wave {
             const int h = lo & 1, bs = lo + h, t = (hi - bs) & 1, be = hi - t;
             const int q0 = bs >> 1, qe = be >> 1;
             // pairs [q0, q1) are in LDS, pairs [qg, qe) in the global slot (split schedule; empty otherwise)
-            const int qR = rl >> 1, q1 = min(qe, qR), qg = max(q0, qR);
+            const int qR = DMI_SLOT_ROWS ? (rl >> 1) : qe, q1 = min(qe, qR), qg = max(q0, qR);
             double s0 = cj[j1], s1 = 0.0, s2 = 0.0, s3 = 0.0;
             if (h) s1 = cj[lo] * cu[lo];
             if (t) s2 = Aget(hi - 1, j) * Aget(hi - 1, j1);
@@ -701,7 +720,7 @@ struct Wave {
                 s1 = fma(a0.y, u0.y, s1);
             }
             // global pairs continue the same accumulator pattern: pair q goes to (s0, s1) when q - q0 is even
-            if (qg < qe) {
+            if (DMI_SLOT_ROWS && qg < qe) {
                 cgv2 *gj = (cgv2 *)(G + (size_t)j * gld), *gu = (cgv2 *)(G + (size_t)j1 * gld);
                 int r = qg;
                 auto acc = [&](const dv2 &a, const dv2 &u, int pq) {
@@ -757,7 +776,7 @@ struct Wave {
                 w1 = fma(a.y, bb.y, w1);
                 vj[q] = a;
             }
-            if (qg < qe) {
+            if (DMI_SLOT_ROWS && qg < qe) {
                 gv2 *gj = (gv2 *)(G + (size_t)j * gld);
                 cgv2 *gu = (cgv2 *)(G + (size_t)j1 * gld);
                 int r = qg;
@@ -808,7 +827,7 @@ struct Wave {
             if (h) s0 = cj[lo] * b[lo];
             if (t) s1 = Aget(hi - 1, j) * b[hi - 1];
             int q = bs >> 1;
-            const int q0 = q, qe = be >> 1, qR = rl >> 1, q1 = min(qe, qR), qg = max(q0, qR);
+            const int q0 = q, qe = be >> 1, qR = DMI_SLOT_ROWS ? (rl >> 1) : qe, q1 = min(qe, qR), qg = max(q0, qR);
             // four row pairs per step (all eight loads issued before the first use), then two, then one;
             // accumulator k always takes the same row of a step, as a plain two-pair loop would
             for (; q + 4 <= q1; q += 4) {
@@ -828,7 +847,7 @@ struct Wave {
                 s1 = fma(a0.y, u0.y, s1);
             }
             // global pairs (split schedule) continue the pattern: pair q goes to (s0, s1) when q - q0 is even
-            if (qg < qe) {
+            if (DMI_SLOT_ROWS && qg < qe) {
                 cgv2 *gj = (cgv2 *)(G + (size_t)j * gld);
                 auto acc = [&](const dv2 &a, const dv2 &u, int pq) {
                     if (((pq - q0) & 1) == 0) { s0 = fma(a.x, u.x, s0); s1 = fma(a.y, u.y, s1); }
@@ -936,7 +955,7 @@ struct Wave {
                     lds_order();
                 }
                 for (int r = L; r < mrows; r += kWave) {
-                    if (r < rl) {
+                    if (!DMI_SLOT_ROWS || r < rl) {
                         double t = Aat(r, imv);
                         for (int j = imv; j < nsetp - 1; ++j) Aat(r, j) = Aat(r, j + 1);
                         Aat(r, nsetp - 1) = t;
@@ -1130,8 +1149,9 @@ struct Wave {
         if (L + kWave < m) zz[L + kWave] = b0[L + kWave] - a1 * xj;
         // Tikhonov rows below the copied ones are cleared while the copy is in flight; a pad row (odd m) after it
         if (tikh && L < n) {
-            for (int i = ldg; i < min(M, rl); ++i) Aat(i, L) = 0.0;
-            for (int i = max(ldg, rl); i < M; ++i) G[(size_t)L * gld + (i - rl)] = 0.0;
+            for (int i = ldg; i < (DMI_SLOT_ROWS ? min(M, rl) : M); ++i) Aat(i, L) = 0.0;
+            if (DMI_SLOT_ROWS)
+                for (int i = max(ldg, rl); i < M; ++i) G[(size_t)L * gld + (i - rl)] = 0.0;
         }
         basis_wait();
         if (tikh && L < n && ldg > m) Aat(m, L) = 0.0;
@@ -2655,6 +2675,7 @@ __global__ void t2part_kernel(T2partParams p) {
 }  // namespace
 
 // ------------------------------------ host-side launchers ------------------------------------
+#if !DMI_SPLIT_TU
 #ifdef DMI_PHASE_PROFILE
 extern "C" int dmi_phase_read(unsigned long long *out) {
     return (int)hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), sizeof(g_phase));
@@ -2744,7 +2765,22 @@ hipError_t launch_select(const double *image, int64_t v0, int64_t v1, double thr
     return hipGetLastError();
 }
 
+#endif  // !DMI_SPLIT_TU
 using T2mapKernelFn = void (*)(T2mapParams);
+#if DMI_SPLIT_TU
+// split schedules exist for the Tikhonov-regularized methods only (lcurve, chi2, mdp)
+static T2mapKernelFn t2map_kernel_for(int reg, int legacy) {
+    switch (reg + (legacy ? 8 : 0)) {
+    case 1: return t2map_kernel<1>;
+    case 3: return t2map_kernel<3>;
+    case 4: return t2map_kernel<4>;
+    case 9: return t2map_kernel<9>;
+    case 11: return t2map_kernel<11>;
+    case 12: return t2map_kernel<12>;
+    default: return nullptr;
+    }
+}
+#else
 static T2mapKernelFn t2map_kernel_for(int reg, int legacy) {
     if (legacy) {
         switch (reg) {
@@ -2763,13 +2799,36 @@ static T2mapKernelFn t2map_kernel_for(int reg, int legacy) {
     default: return t2map_kernel<4>;
     }
 }
+#endif
 
+#if DMI_SPLIT_TU
+hipError_t launch_t2map_split(const T2mapParams &p, int grid, size_t lds, hipStream_t s) {
+    const T2mapKernelFn k = t2map_kernel_for(p.reg, p.legacy);
+    if (!k) return hipErrorInvalidDeviceFunction;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(64), lds, s, p);
+    return hipGetLastError();
+}
+
+hipError_t t2map_occupancy_split(int reg, int legacy, int *blocks_per_cu, size_t lds) {
+    const T2mapKernelFn k = t2map_kernel_for(reg, legacy);
+    if (!k) return hipErrorInvalidDeviceFunction;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, 64, lds);
+}
+#else
 hipError_t launch_t2map(const T2mapParams &p, int grid, size_t lds, hipStream_t s) {
+#if !DMI_SLOT_ROWS
+    if (p.rl < p.mw) return launch_t2map_split(p, grid, lds, s);
+#endif
     hipLaunchKernelGGL(t2map_kernel_for(p.reg, p.legacy), dim3(grid), dim3(64), lds, s, p);
     return hipGetLastError();
 }
 
-hipError_t t2map_occupancy(int reg, int legacy, int *blocks_per_cu, size_t lds) {
+hipError_t t2map_occupancy(int reg, int legacy, int *blocks_per_cu, size_t lds, bool split) {
+#if !DMI_SLOT_ROWS
+    if (split) return t2map_occupancy_split(reg, legacy, blocks_per_cu, lds);
+#else
+    (void)split;
+#endif
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, t2map_kernel_for(reg, legacy), 64, lds);
 }
 
@@ -2808,4 +2867,5 @@ hipError_t launch_t2part(const T2partParams &p, hipStream_t s) {
     return hipGetLastError();
 }
 
+#endif  // DMI_SPLIT_TU
 }  // namespace dmi

@@ -15,7 +15,9 @@ RUN = PKG / "dmi_run"
 # Same sources built with -DDMI_PHASE_PROFILE: per-phase shader-clock counters (decaes_amd.phase_profile)
 PHASE_LIB = PKG / "libdecaes_amd_phase.so"
 
-SOURCES = ["dmi_kernels.hip", "dmi_host.cpp", "dmi_api.cpp", "dmi_server.cpp", "dmi_synth.cpp"]
+# dmi_kernels_split.hip: the voxel kernel of the split schedule (dmi_kernels.hip compiled once more); the instrumented
+# build keeps both schedules in dmi_kernels.hip
+SOURCES = ["dmi_kernels.hip", "dmi_kernels_split.hip", "dmi_host.cpp", "dmi_api.cpp", "dmi_server.cpp", "dmi_synth.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
@@ -41,6 +43,8 @@ def build(verbose: bool = False, phase_profile: bool = False) -> Path:
     out = PHASE_LIB if phase_profile else LIB
     extra = ["-DDMI_PHASE_PROFILE"] if phase_profile else []
     for src in SOURCES:
+        if phase_profile and src == "dmi_kernels_split.hip":
+            continue
         obj = obj_dir / (src + ".o")
         cmd = [hipcc(), *HIPCC_FLAGS, *extra, "-c", "-I", str(REPO / "include"), "-I", str(CSRC), str(CSRC / src),
                "-o", str(obj)]
