@@ -3,6 +3,7 @@ from __future__ import annotations
 
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -37,7 +38,7 @@ def hipcc() -> str:
 def build(verbose: bool = False, phase_profile: bool = False) -> Path:
     """Compile every source into decaes_amd/libdecaes_amd.so (always recompiles). With phase_profile, build the
     instrumented variant decaes_amd/libdecaes_amd_phase.so instead."""
-    objs = []
+    objs, cmds = [], []
     obj_dir = ROOT / ("_build_phase" if phase_profile else "_build")
     obj_dir.mkdir(exist_ok=True)
     out = PHASE_LIB if phase_profile else LIB
@@ -52,8 +53,13 @@ def build(verbose: bool = False, phase_profile: bool = False) -> Path:
             cmd = [c for c in cmd if c not in ("--offload-arch=gfx950",)]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append(cmd)
         objs.append(str(obj))
+    # the sources compile independently: one hipcc process each, at most MAX_JOBS (or the CPU count) at once
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", 0)) or os.cpu_count() or 1))
+    with ThreadPoolExecutor(jobs) as pool:
+        for f in [pool.submit(subprocess.run, cmd, check=True) for cmd in cmds]:
+            f.result()
     tmp = out.with_suffix(".so.tmp")
     link = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp), "-lpthread"]
     if verbose:
