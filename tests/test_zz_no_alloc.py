@@ -131,6 +131,8 @@ def test_noalloc_graph_replays_match_eager_calls(dmi, name):
         assert np.array_equal(v.cpu().numpy(), ref2[k], equal_nan=True), k
 
 
+# the refusal comes before anything is recorded, so torch reports the captured graph as empty
+@pytest.mark.filterwarnings("ignore:The CUDA Graph is empty")
 def test_capture_needs_the_grids_built_outside_capture(dmi):
     """Under capture, options whose grids are not in the static scratch are refused before anything is recorded."""
     import torch
