@@ -14,6 +14,8 @@ stream below capacity, the roofline object and the CPU baseline.
     python bench.py --copy-probe BYTES    # plain device copy of known size, the HBM-counter calibration run
     python bench.py --serve [--qps 2,8,32] [--requests 16] [--max-slices 8]   # served stream: p50/p99 latency
     python bench.py --batch [--volumes 64]   # batched uneven workload of seeded volumes through the request loop
+    python bench.py --cli [--workload c2]    # the command line end to end: NIfTI in, MAT v7.3 out
+    python bench.py --host [--workload c2] --precision fp64 --steps 3 --warmup 1   # host arrays, PCIe included
 """
 from __future__ import annotations
 
@@ -362,6 +364,48 @@ def cli_run(args):
     print(json.dumps(res), flush=True)
 
 
+def host_run(args):
+    """The host-array entry point (dmi_t2map_f64 through T2mapSEcorr) on the workload's whole volume, PCIe copies
+    included: the image goes in and the maps come back on every call. The context is created before the warmup calls
+    and each call's wall time is taken around T2mapSEcorr, beside the library's own kernel and call totals."""
+    import numpy as np
+
+    import decaes_amd as dmi
+
+    w = WORKLOADS[args.workload]
+    img, nin = dmi.synth_mse(w["shape"], w["nTE"], w["TE"], ncomp=w["ncomp"], mask_fraction=w["mask"], seed=2024)
+    extra = {} if w["Chi2Factor"] is None else {"Chi2Factor": w["Chi2Factor"]}
+    opts = dmi.T2mapOptions.from_image(img, TE=w["TE"], nT2=w["nT2"], T2Range=w["T2Range"], Reg=w["Reg"],
+                                       nRefAngles=w["nRefAngles"], legacy=bool(w.get("legacy")),
+                                       precision=args.precision, Silent=True, **extra)
+    t = time.perf_counter()
+    ctx = dmi.context(0)
+    ctx_s = time.perf_counter() - t
+    walls, kms, tms, out_bytes = [], [], [], 0
+    for i in range(args.warmup + args.steps):
+        t = time.perf_counter()
+        maps, dist = dmi.T2mapSEcorr(img, opts)
+        wall = time.perf_counter() - t
+        st = ctx.stats()
+        phase = "warmup" if i < args.warmup else "step"
+        log(f"[host] {phase} {i}: wall {wall * 1e3:.1f} ms, library {st['total_ms']:.1f} ms, kernel "
+            f"{st['kernel_ms']:.1f} ms, {st['voxels']} voxels")
+        if i >= args.warmup:
+            walls.append(wall)
+            kms.append(st["kernel_ms"])
+            tms.append(st["total_ms"])
+        out_bytes = dist.nbytes + sum(v.nbytes for k, v in maps.items() if v.shape[:3] == tuple(w["shape"]))
+        del maps, dist
+    wall = float(np.mean(walls))
+    res = {"host": {"workload": args.workload, "precision": args.precision, "voxels_in_mask": nin,
+                    "image_bytes": img.nbytes, "output_bytes": out_bytes, "context_s": ctx_s, "calls": len(walls),
+                    "wall_ms_per_call": wall * 1e3, "library_ms_per_call": float(np.mean(tms)),
+                    "kernel_ms_per_call": float(np.mean(kms)), "voxels_per_s_pcie_inclusive": nin / wall,
+                    "voxels_per_s_kernel": nin / (float(np.mean(kms)) * 1e-3),
+                    "schedule": {"lds_rows": st["lds_rows"], "workgroups_per_cu": st["workgroups_per_cu"]}}}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -391,6 +435,8 @@ def main():
     ap.add_argument("--cli", action="store_true",
                     help="the command line end to end on the workload's volume written as NIfTI (load, T2map, "
                          "T2part, MAT v7.3 output)")
+    ap.add_argument("--host", action="store_true",
+                    help="the host-array entry point on the workload's whole volume, PCIe copies included")
     args = ap.parse_args()
     # PyTorch's HIP runtime is loaded before the library, which then binds to it (one runtime per process)
     import torch  # noqa: F401
@@ -409,6 +455,8 @@ def main():
         return batch(args)
     if args.cli:
         return cli_run(args)
+    if args.host:
+        return host_run(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); voxels shard across GPUs with no exchange "
               "step, so a multi-GPU figure is N independent single-GPU runs.", file=sys.stderr)
