@@ -545,7 +545,8 @@ int dmi_create(dmi_ctx **out, int device, uint32_t flags) {
     auto *c = new dmi_ctx();
     c->device = device;
     c->num_cu = prop.multiProcessorCount;
-    if ((e = hipSetDevice(device)) != hipSuccess || (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
+    dmi_host::DeviceGuard guard(device);
+    if ((e = guard.status) != hipSuccess || (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
         delete c;
         return dmi_host::errorf(g_create_error, DMI_EHIP, "stream creation failed: %s", hipGetErrorString(e));
     }
@@ -563,7 +564,7 @@ int dmi_create(dmi_ctx **out, int device, uint32_t flags) {
 
 int dmi_destroy(dmi_ctx *c) {
     if (!c) return DMI_OK;
-    (void)hipSetDevice(c->device);
+    dmi_host::DeviceGuard guard(c->device);
     if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
     c->grids.release();
     c->lb.release();
@@ -616,7 +617,7 @@ int dmi_t2map_f64_device(dmi_ctx *c, const double *d_image, const dmi_t2map_opts
                          const dmi_t2map_out *d_out) {
     if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
-    HIPCHECK(c, hipSetDevice(c->device));
+    DMI_ON_DEVICE(c->err, c->device);
     int rc = dmi_host::validate_t2map(c->err, o);
     if (rc) return rc;
     if (!d_image) return fail(c, DMI_EINVAL, "image is NULL");
@@ -627,7 +628,7 @@ int dmi_t2map_f64_device_noalloc(dmi_ctx *c, const double *d_image, const dmi_t2
                                  const double *d_alpha_in, const dmi_t2map_out *d_out) {
     if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
-    HIPCHECK(c, hipSetDevice(c->device));
+    DMI_ON_DEVICE(c->err, c->device);
     int rc = dmi_host::validate_t2map(c->err, o);
     if (rc) return rc;
     if (!d_image) return fail(c, DMI_EINVAL, "image is NULL");
@@ -638,7 +639,7 @@ int dmi_t2map_f64(dmi_ctx *c, const double *image, const dmi_t2map_opts *o, cons
                   const dmi_t2map_out *out) {
     if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
-    HIPCHECK(c, hipSetDevice(c->device));
+    DMI_ON_DEVICE(c->err, c->device);
     if (int rc0 = no_capture(c, "dmi_t2map_f64")) return rc0;
     int rc = dmi_host::validate_t2map(c->err, o);
     if (rc) return rc;
@@ -713,7 +714,7 @@ int dmi_t2map_f32(dmi_ctx *c, const float *image, const dmi_t2map_opts *o, const
                   const dmi_t2map_out_f32 *out) {
     if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
-    HIPCHECK(c, hipSetDevice(c->device));
+    DMI_ON_DEVICE(c->err, c->device);
     if (int rc0 = no_capture(c, "dmi_t2map_f32")) return rc0;
     int rc = dmi_host::validate_t2map(c->err, o);
     if (rc) return rc;
@@ -750,7 +751,7 @@ int dmi_t2part_f32(dmi_ctx *c, const float *dist, const dmi_t2part_opts *o, floa
                    float *mgm) {
     if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
-    HIPCHECK(c, hipSetDevice(c->device));
+    DMI_ON_DEVICE(c->err, c->device);
     if (int rc0 = no_capture(c, "dmi_t2part_f32")) return rc0;
     if (!o) return fail(c, DMI_EINVAL, "opts is NULL");
     if (!dist || !sfr || !sgm || !mfr || !mgm) return fail(c, DMI_EINVAL, "dist and all four outputs are required");
@@ -771,7 +772,7 @@ int dmi_t2part_f32(dmi_ctx *c, const float *dist, const dmi_t2part_opts *o, floa
 int dmi_epg_basis_set(dmi_ctx *c, const dmi_t2map_opts *o, double *out) {
     if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
-    HIPCHECK(c, hipSetDevice(c->device));
+    DMI_ON_DEVICE(c->err, c->device);
     if (int rc0 = no_capture(c, "dmi_epg_basis_set")) return rc0;
     int rc = dmi_host::validate_t2map(c->err, o);
     if (rc) return rc;
@@ -793,7 +794,7 @@ int dmi_t2part_f64_device(dmi_ctx *c, const double *d_dist, const dmi_t2part_opt
                           double *mfr, double *mgm) {
     if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
-    HIPCHECK(c, hipSetDevice(c->device));
+    DMI_ON_DEVICE(c->err, c->device);
     if (!d_dist || !sfr || !sgm || !mfr || !mgm) return fail(c, DMI_EINVAL, "dist and all four outputs are required");
     return run_t2part(c, d_dist, o, sfr, sgm, mfr, mgm);
 }
@@ -802,7 +803,7 @@ int dmi_t2part_f64(dmi_ctx *c, const double *dist, const dmi_t2part_opts *o, dou
                    double *mgm) {
     if (!c) return fail(nullptr, DMI_EINVAL, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
-    HIPCHECK(c, hipSetDevice(c->device));
+    DMI_ON_DEVICE(c->err, c->device);
     if (int rc0 = no_capture(c, "dmi_t2part_f64")) return rc0;
     if (!o) return fail(c, DMI_EINVAL, "opts is NULL");
     if (!dist || !sfr || !sgm || !mfr || !mgm) return fail(c, DMI_EINVAL, "dist and all four outputs are required");

@@ -22,6 +22,30 @@ int errorf(std::string &err, int code, const char *fmt, ...);
         if (_e != hipSuccess) return ::dmi_host::errorf((err), DMI_EHIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
     } while (0)
 
+// Makes `device` the calling thread's current device for the guard's lifetime and restores the thread's previous
+// device at scope exit: an entry point leaves its caller's current device as it found it.
+struct DeviceGuard {
+    int prev = -1;
+    hipError_t status;
+    explicit DeviceGuard(int device) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        status = hipSetDevice(device);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+    DeviceGuard(const DeviceGuard &) = delete;
+    DeviceGuard &operator=(const DeviceGuard &) = delete;
+};
+
+// Declares the guard of an entry point on `device`; returns DMI_EHIP with the message in `err` if the device cannot be
+// made current.
+#define DMI_ON_DEVICE(err, device)                                                                                 \
+    ::dmi_host::DeviceGuard dmi_device_guard_(device);                                                             \
+    if (dmi_device_guard_.status != hipSuccess)                                                                    \
+        return ::dmi_host::errorf((err), DMI_EHIP, "hipSetDevice(%d) failed: %s", (int)(device),                   \
+                                  hipGetErrorString(dmi_device_guard_.status))
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;

@@ -563,7 +563,8 @@ int dmi_server_create(dmi_server **out, int device, const dmi_server_config *cfg
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return DMI_EHIP;
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return DMI_ENODEV;
-    if (hipSetDevice(device) != hipSuccess) return DMI_EHIP;
+    dmi_host::DeviceGuard guard(device);
+    if (guard.status != hipSuccess) return DMI_EHIP;
     auto S = std::make_unique<dmi_server>();
     S->device = device;
     S->num_cu = prop.multiProcessorCount;
@@ -599,7 +600,7 @@ int dmi_server_destroy(dmi_server *S) {
     }
     if (S->sched.joinable()) S->sched.join();
     if (S->copier.joinable()) S->copier.join();
-    (void)hipSetDevice(S->device);
+    dmi_host::DeviceGuard guard(S->device);
     for (auto &st : S->streams) {
         if (st.s) (void)hipStreamSynchronize(st.s);
         for (auto &c : st.inflight) (void)hipEventDestroy(c.ev);
