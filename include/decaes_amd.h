@@ -102,7 +102,8 @@ typedef struct {
 } dmi_stats;
 
 /* Context: one HIP stream on `device`, reusable device buffers and cached grid bases.
- * Replaces the per-task thread_buffer_maker (src/T2mapSEcorr.jl:596-614). flags: reserved, pass 0. */
+ * Replaces the per-task thread_buffer_maker (src/T2mapSEcorr.jl:596-614). flags: reserved, pass 0.
+ * Every call on a context runs with its device current and restores the calling thread's current device on return. */
 int dmi_create(dmi_ctx **out, int device, uint32_t flags);
 int dmi_destroy(dmi_ctx *ctx);
 /* Runs ctx's later calls on `stream` (a hipStream_t of ctx's device), or again on ctx's own stream when stream is
