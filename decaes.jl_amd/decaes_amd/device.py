@@ -94,8 +94,9 @@ def t2part_device(dist, opts: T2partOptions, out: dict, ctx: Optional[_lib.Conte
 
 def capture(image, opts: T2mapOptions, out: dict, alpha=None, t2part=None, ctx: Optional[_lib.Context] = None):
     """T2mapSEcorr (dmi_t2map_f64_device_noalloc) and, with t2part = (T2partOptions, outputs), T2partSEcorr on its
-    distribution, recorded into a torch.cuda.CUDAGraph. Each graph.replay() fills the outputs with NaN and recomputes
-    them from the current contents of `image` (and `alpha`), as fresh calls on NaN-filled outputs would.
+    distribution, recorded into a torch.cuda.CUDAGraph. Each graph.replay() fills the outputs with NaN (and
+    out["alpha"] with the B1 map `alpha`, when given, as T2mapSEcorr's load_B1map! does, src/T2mapSEcorr.jl:56-61)
+    and recomputes them from the current contents of `image` (and `alpha`), as fresh calls would.
 
     The calls run once outside capture first: that puts the grids of `opts` in the device's static scratch and the
     T2part tables on the context, which the graph reads at every replay. A later call with other options on the same
@@ -110,8 +111,11 @@ def capture(image, opts: T2mapOptions, out: dict, alpha=None, t2part=None, ctx: 
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g):
         s = torch.cuda.current_stream(image.device)
-        for t in list(out.values()) + (list(t2part[1].values()) if t2part is not None else []):
-            t.fill_(float("nan"))
+        for k, t in list(out.items()) + (list(t2part[1].items()) if t2part is not None else []):
+            if alpha is not None and k == "alpha" and t is out.get("alpha"):
+                t.copy_(alpha)
+            else:
+                t.fill_(float("nan"))
         t2map_device(image, opts, out, alpha, ctx, noalloc=True, stream=s)
         if t2part is not None:
             t2part_device(out["dist"], t2part[0], t2part[1], ctx, stream=s)

@@ -72,11 +72,13 @@ def _t2part_outputs(torch, opts):
 
 
 GRAPH_CASES = {
-    # shape, nTE, TE, nT2, T2Range, Reg, nRefAngles, extra options, mask fraction
+    # shape, nTE, TE, nT2, T2Range, Reg, nRefAngles, extra options, mask fraction, B1 map
     # two selection chunks: two selections and two voxel-kernel launches that read the selected count on the device
-    "gcv_two_chunks": ((96, 96, 32), 48, 10e-3, 40, (10e-3, 2.0), "gcv", 32, {}, 0.3),
+    "gcv_two_chunks": ((96, 96, 32), 48, 10e-3, 40, (10e-3, 2.0), "gcv", 32, {}, 0.3, False),
     # the split schedule: workspace rows beyond the LDS part in the workgroups' global slots
-    "lcurve_split": ((60, 60, 4), 56, 7.9e-3, 60, (8e-3, 2.0), "lcurve", 64, dict(schedule="split"), 0.8),
+    "lcurve_split": ((60, 60, 4), 56, 7.9e-3, 60, (8e-3, 2.0), "lcurve", 64, dict(schedule="split"), 0.8, False),
+    # a B1 map: maps["alpha"] holds the map on every voxel, at or below the threshold too, at every replay
+    "chi2_b1": ((48, 48, 8), 32, 10e-3, 40, (10e-3, 2.0), "chi2", 24, dict(Chi2Factor=1.02), 0.5, True),
 }
 
 
@@ -89,17 +91,22 @@ def test_noalloc_graph_replays_match_eager_calls(dmi, name):
     import torch
     from decaes_amd import device as D
 
-    shape, m, TE, n, T2R, reg, K, extra, frac = GRAPH_CASES[name]
+    shape, m, TE, n, T2R, reg, K, extra, frac, b1 = GRAPH_CASES[name]
     print(f"[graph] case={name} shape={shape} nTE={m} {reg}", flush=True)
     img, nin = dmi.synth_mse(shape, m, TE, mask_fraction=frac, seed=37)
     opts = dmi.T2mapOptions.from_image(img, TE=TE, nT2=n, T2Range=T2R, Reg=reg, nRefAngles=K, SaveRegParam=True,
                                        SaveResidualNorm=True, SaveDecayCurve=True, Silent=True, **extra)
     popts = dmi.T2partOptions.from_t2map(opts, SPWin=(10e-3, 40e-3), MPWin=(40e-3, 2.0))
     dimg = torch.from_numpy(np.ascontiguousarray(img.transpose(3, 2, 1, 0))).to("cuda")
+    dalpha = None
+    if b1:
+        dalpha = torch.from_numpy(np.random.default_rng(5).uniform(130, 180, shape[::-1])).to("cuda")
 
     def eager():
         out, parts = D.alloc_t2map_outputs(opts, device="cuda"), _t2part_outputs(torch, opts)
-        s = D.t2map_device(dimg, opts, out, noalloc=True)
+        if b1:
+            out["alpha"].copy_(dalpha)
+        s = D.t2map_device(dimg, opts, out, alpha=dalpha, noalloc=True)
         D.t2part_device(out["dist"], popts, parts)
         torch.cuda.synchronize()
         return s, {k: v.cpu().numpy() for k, v in {**out, **parts}.items()}
@@ -108,8 +115,10 @@ def test_noalloc_graph_replays_match_eager_calls(dmi, name):
     assert s_ref["voxels"] == nin
     if extra.get("schedule") == "split":
         assert s_ref["lds_rows"] < m + n
+    if b1:
+        assert nin < np.prod(shape) and np.array_equal(ref["alpha"], dalpha.cpu().numpy())
     out, parts = D.alloc_t2map_outputs(opts, device="cuda"), _t2part_outputs(torch, opts)
-    g = D.capture(dimg, opts, out, t2part=(popts, parts))
+    g = D.capture(dimg, opts, out, alpha=dalpha, t2part=(popts, parts))
     other = torch.linspace(0.0, 1.0, 1 << 20, dtype=torch.float64, device="cuda")
     for r in range(3):
         print(f"[graph] replay {r}", flush=True)
