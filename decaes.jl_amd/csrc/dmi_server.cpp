@@ -558,6 +558,7 @@ extern "C" {
 int dmi_server_create(dmi_server **out, int device, const dmi_server_config *cfg) {
     if (!out) return DMI_EINVAL;
     *out = nullptr;
+    if (cfg && cfg->order != DMI_ORDER_SHORTEST && cfg->order != DMI_ORDER_TURNS) return DMI_EINVAL;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev) return DMI_ENODEV;
     hipDeviceProp_t prop;
@@ -571,17 +572,26 @@ int dmi_server_create(dmi_server **out, int device, const dmi_server_config *cfg
     if (cfg) S->cfg = *cfg;
     if (S->cfg.max_streams <= 0) S->cfg.max_streams = 4;
     if (S->cfg.chunk_voxels <= 0) S->cfg.chunk_voxels = 65536;
-    if (S->cfg.order != DMI_ORDER_TURNS) S->cfg.order = DMI_ORDER_SHORTEST;
     if (S->cfg.hbm_budget_bytes <= 0) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) return DMI_EHIP;
         S->cfg.hbm_budget_bytes = (int64_t)(fr / 2);
     }
-    if (hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking) != hipSuccess) return DMI_EHIP;
+    // on a failure, the streams and pinned slots created so far are released before returning
+    auto undo = [&S]() {
+        for (auto &st : S->streams) {
+            if (st.h_status) (void)hipHostFree(st.h_status);
+            if (st.s) (void)hipStreamDestroy(st.s);
+        }
+        if (S->copy) (void)hipStreamDestroy(S->copy);
+        return DMI_EHIP;
+    };
+    if (hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking) != hipSuccess) return undo();
     S->streams.resize(S->cfg.max_streams);
     for (auto &st : S->streams) {
-        if (hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking) != hipSuccess) return DMI_EHIP;
-        if (hipHostMalloc((void **)&st.h_status, sizeof(int) * kMaxInflight, hipHostMallocDefault) != hipSuccess) return DMI_EHIP;
+        if (hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking) != hipSuccess) return undo();
+        if (hipHostMalloc((void **)&st.h_status, sizeof(int) * kMaxInflight, hipHostMallocDefault) != hipSuccess)
+            return undo();
     }
     dmi_server *raw = S.release();
     raw->sched = std::thread(sched_loop, raw);

@@ -187,7 +187,7 @@ typedef struct {
     int64_t hbm_budget_bytes; /* device memory for admitted requests; <= 0 selects half of the free memory */
     int32_t max_streams;      /* option sets in flight at once, one HIP stream each; <= 0 selects 4 */
     int32_t chunk_voxels;     /* voxels per launch; <= 0 selects 65536 */
-    int32_t order;            /* which admitted request of a stream gets the next chunk: DMI_ORDER_* */
+    int32_t order;            /* which admitted request of a stream gets the next chunk: DMI_ORDER_* (else DMI_EINVAL) */
     int32_t reserved;         /* pass 0 */
 } dmi_server_config;
 

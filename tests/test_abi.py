@@ -98,3 +98,13 @@ def test_server_config_layout_and_order_check():
                                                            "reserved"]
     with pytest.raises(ValueError, match="order"):
         Server(order="fifo")
+    # the C entry point rejects an unknown order before it looks for a device; a valid one gets that far
+    L = _lib.load()
+    h = C.c_void_p()
+    assert L.dmi_server_create(C.byref(h), 0, C.byref(_lib.ServerConfigC(0, 0, 0, 7, 0))) == _lib.DMI_EINVAL
+    assert not h.value
+    for order in (0, 1):
+        rc = L.dmi_server_create(C.byref(h), 0, C.byref(_lib.ServerConfigC(0, 0, 0, order, 0)))
+        assert rc != _lib.DMI_EINVAL
+        if rc == _lib.DMI_OK:
+            assert L.dmi_server_destroy(h) == _lib.DMI_OK
