@@ -31,8 +31,12 @@ static int write_f64(const char *dir, const char *name, const double *p, size_t 
         return 1;
     }
     size_t w = fwrite(p, sizeof(double), n, f);
-    fclose(f);
-    return w == n ? 0 : 1;
+    const int closed = fclose(f) == 0;
+    if (w != n || !closed) {
+        fprintf(stderr, "dmi_run: short write to %s\n", path);
+        return 1;
+    }
+    return 0;
 }
 
 static double *nan_array(size_t n) {
@@ -97,6 +101,10 @@ int main(int argc, char **argv) {
     double *image = (double *)malloc(sizeof(double) * V * (size_t)nTE);
     if (!image) return 1;
     const int64_t nin = dmi_synth_mse(image, X, Y, Z, nTE, TE, o.T1, 3, frac, seed, 8);
+    if (nin < 0) {
+        fprintf(stderr, "dmi_run: dmi_synth_mse rejected the sizes\n");
+        return 2;
+    }
 
     dmi_ctx *ctx = NULL;
     int rc = dmi_create(&ctx, 0, 0);
