@@ -30,7 +30,8 @@ from pathlib import Path
 
 import numpy as np
 
-OUT = Path(os.environ.get("GRAFT_REPO_ROOT", Path(__file__).resolve().parents[1])) / "gpurun_out"
+# folder for each case's full per-voxel record (parity_<case>.json); unset: the summary line is printed only
+OUT = os.environ.get("DMI_PARITY_DUMPS")
 
 SCALES = tuple(1.0 + j * 2.0 ** -52 for k in range(1, 9) for j in (k, -k))
 ORDERED = tuple((order, 1.0 + j * 2.0 ** -52) for order in (1, 2) for j in (0, 1, -1))  # (sum order, scale)
@@ -132,9 +133,10 @@ def check(name, oracle, T2R, sel, spread, maps, dist, omaps, odist, *, derived=(
         s["stable_mu_rel"] = q(mu_rel[stable])
     bad = lin[stable & ((rel > BAR_DIST) | (da > BAR_ALPHA) | (mwf > BAR_MWF))]
     s["stable_failures"] = [int(i) for i in bad]
-    OUT.mkdir(exist_ok=True)
-    with open(OUT / f"parity_{name}.json", "w") as f:
-        json.dump(s, f, indent=1)
+    if OUT:
+        Path(OUT).mkdir(parents=True, exist_ok=True)
+        with open(Path(OUT) / f"parity_{name}.json", "w") as f:
+            json.dump(s, f, indent=1)
     print(json.dumps({k: v for k, v in s.items() if not k.startswith("unstable_")} |
                      dict(unstable=len(unstable_ids), unstable_max_dist=float(rel[~stable].max(initial=0.0)))),
           flush=True)

@@ -7,6 +7,7 @@ oracle's own spread under ulp-scaled inputs (parity_util).
 """
 import json
 import os
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -220,9 +221,10 @@ def test_mixed_precision_error(dmi, oracle, name):
 
     s = dict(case=f"mixed_{name}", voxels=int(mask.sum()), alpha_abs_deg=q(da), dist_rel_l2=q(rel), mwf_abs=q(mwf),
              sgm_rel=q(sgm), chi2factor_abs=q(chi))
-    OUT.mkdir(exist_ok=True)
-    with open(OUT / f"parity_mixed_{name}.json", "w") as f:
-        json.dump(s, f, indent=1)
+    if OUT:
+        Path(OUT).mkdir(parents=True, exist_ok=True)
+        with open(Path(OUT) / f"parity_mixed_{name}.json", "w") as f:
+            json.dump(s, f, indent=1)
     print(json.dumps(s), flush=True)
     assert np.max(da) <= 0.1, s
     assert np.max(mwf) <= 1e-3, s
