@@ -35,6 +35,35 @@ def alloc_t2map_outputs(opts: T2mapOptions, device="cuda"):
     return out
 
 
+def _check(items, device: int):
+    """items: (name, tensor, numel). The library reads or writes exactly `numel` doubles at each tensor's address on
+    `device`: anything else is refused here, before a kernel could touch memory outside a tensor. Sizes are checked
+    for every tensor before devices."""
+    for name, t, numel in items:
+        if str(t.dtype) != "torch.float64" or not t.is_contiguous() or t.numel() != numel:
+            raise ValueError(f"{name} must be a contiguous float64 tensor of {numel} elements, got {t.dtype} "
+                             f"{tuple(t.shape)}{'' if t.is_contiguous() else ' (not contiguous)'}")
+    for name, t, _ in items:
+        if not t.is_cuda or (t.device.index or 0) != device:
+            raise ValueError(f"{name} must be on cuda:{device}, got {t.device}")
+
+
+def _check_t2map(image, opts: T2mapOptions, out: dict, alpha, device: int):
+    X, Y, Z = opts.MatrixSize
+    V = X * Y * Z
+    if "dist" not in out:
+        raise ValueError('out["dist"] is required')
+    items = [("image", image, V * opts.nTE)]
+    if alpha is not None:
+        items.append(("alpha", alpha, V))
+    sizes = dict(dist=V * opts.nT2, decaycurve=V * opts.nTE,
+                 decaybasis=opts.nTE * opts.nT2 * (V if opts.SetFlipAngle is None else 1))
+    for k, t in out.items():
+        if t is not None and k in ("dist", "gdn", "ggm", "gva", "fnr", "snr", "alpha", *_OPTIONAL):
+            items.append((f'out["{k}"]', t, sizes.get(k, V)))
+    _check(items, device)
+
+
 def _ready(t):
     """The library runs on its own HIP stream: work torch queued on its current stream for the arrays (an input
     computed just before the call, outputs just filled) must be finished before the library reads them."""
@@ -68,7 +97,9 @@ def t2map_device(image, opts: T2mapOptions, out: dict, alpha=None, ctx: Optional
     in the call); returns the call's device counters (dmi_last_stats). With `stream` (a torch.cuda.Stream) the call
     runs on that stream, after the work queued on it; while that stream is captured into a graph, a noalloc call
     records its launches there."""
-    ctx = ctx or _lib.context(image.device.index or 0)
+    dev = ctx.device if ctx is not None else (image.device.index or 0)
+    _check_t2map(image, opts, out, alpha, dev)
+    ctx = ctx or _lib.context(dev)
     if stream is None:
         _ready(image)
     oc = t2map_opts_c(opts)
@@ -83,7 +114,11 @@ def t2map_device(image, opts: T2mapOptions, out: dict, alpha=None, ctx: Optional
 
 def t2part_device(dist, opts: T2partOptions, out: dict, ctx: Optional[_lib.Context] = None, stream=None) -> None:
     """dmi_t2part_f64_device on torch tensors; `stream` as for t2map_device."""
-    ctx = ctx or _lib.context(dist.device.index or 0)
+    dev = ctx.device if ctx is not None else (dist.device.index or 0)
+    X, Y, Z = opts.MatrixSize
+    parts = [(f'out["{k}"]', out[k], X * Y * Z) for k in ("sfr", "sgm", "mfr", "mgm")]
+    _check([("dist", dist, X * Y * Z * opts.nT2)] + parts, dev)
+    ctx = ctx or _lib.context(dev)
     if stream is None:
         _ready(dist)
     oc = t2part_opts_c(opts)
