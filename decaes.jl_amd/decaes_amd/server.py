@@ -35,6 +35,31 @@ class Request:
     done: bool = False
 
 
+def request_arrays(image: np.ndarray, opts: T2mapOptions, alpha: Optional[np.ndarray] = None):
+    """The host arrays of one request -> (image, alpha, maps, dist): the image and B1 map as column-major FP64 of the
+    options' sizes (the library reads exactly that many doubles from each), and the NaN-filled outputs, with the B1
+    map in maps["alpha"] (load_B1map!, src/T2mapSEcorr.jl:56-61)."""
+    image = np.asfortranarray(image, dtype=np.float64)
+    assert image.shape == (*opts.MatrixSize, opts.nTE), f"image shape {image.shape} != {(*opts.MatrixSize, opts.nTE)}"
+    shape3 = opts.MatrixSize
+    out = {k: _nan(shape3) for k in ("gdn", "ggm", "gva", "fnr", "snr", "alpha")}
+    if alpha is not None:
+        alpha = np.asfortranarray(alpha, dtype=np.float64)
+        assert alpha.shape == tuple(shape3), f"B1 map shape {alpha.shape} != {tuple(shape3)}"
+        out["alpha"][...] = alpha
+    if opts.SaveResidualNorm:
+        out["resnorm"] = _nan(shape3)
+    if opts.SaveDecayCurve:
+        out["decaycurve"] = _nan((*shape3, opts.nTE))
+    if opts.SaveRegParam:
+        out["mu"] = _nan(shape3)
+        out["chi2factor"] = _nan(shape3)
+    if opts.SaveNNLSBasis:
+        out["decaybasis"] = (_nan((*shape3, opts.nTE, opts.nT2)) if opts.SetFlipAngle is None
+                             else np.empty((opts.nTE, opts.nT2), order="F"))
+    return image, alpha, out, _nan((*shape3, opts.nT2))
+
+
 class Server:
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, max_streams: int = 0, chunk_voxels: int = 0,
                  order: str = "shortest"):
@@ -59,24 +84,8 @@ class Server:
 
     def submit(self, image: np.ndarray, opts: T2mapOptions, part_opts: Optional[T2partOptions] = None,
                alpha: Optional[np.ndarray] = None) -> Request:
-        image = np.asfortranarray(image, dtype=np.float64)
-        assert image.shape == (*opts.MatrixSize, opts.nTE)
+        image, alpha, out, dist = request_arrays(image, opts, alpha)
         shape3 = opts.MatrixSize
-        out = {k: _nan(shape3) for k in ("gdn", "ggm", "gva", "fnr", "snr", "alpha")}
-        if alpha is not None:
-            alpha = np.asfortranarray(alpha, dtype=np.float64)
-            out["alpha"][...] = alpha
-        if opts.SaveResidualNorm:
-            out["resnorm"] = _nan(shape3)
-        if opts.SaveDecayCurve:
-            out["decaycurve"] = _nan((*shape3, opts.nTE))
-        if opts.SaveRegParam:
-            out["mu"] = _nan(shape3)
-            out["chi2factor"] = _nan(shape3)
-        if opts.SaveNNLSBasis:
-            out["decaybasis"] = (_nan((*shape3, opts.nTE, opts.nT2)) if opts.SetFlipAngle is None
-                                 else np.empty((opts.nTE, opts.nT2), order="F"))
-        dist = _nan((*shape3, opts.nT2))
         oc = t2map_opts_c(opts)
         oo = _lib.T2mapOutC(_ptr(dist), *(_ptr(out.get(k)) for k in (
             "gdn", "ggm", "gva", "fnr", "snr", "alpha", "resnorm", "decaycurve", "mu", "chi2factor", "decaybasis")))

@@ -1,6 +1,6 @@
 """The device-array entry points (decaes_amd.device) refuse tensors whose dtype, layout, size or device differ from
-what the kernels read and write. They do so on the host, before any library call, so these run without a GPU: every
-size here is checked before any device."""
+what the kernels read and write, and the request loop refuses host arrays of other sizes than its options'. Both do
+so on the host, before any library call, so these run without a GPU: every size here is checked before any device."""
 import pytest
 
 torch = pytest.importorskip("torch")
@@ -77,3 +77,23 @@ def test_t2part_device_refuses_mismatched_tensors(dmi):
         D.t2part_device(torch.zeros(V * 40, dtype=torch.float64), po, {**parts, "mfr": torch.zeros(V - 2)})
     with pytest.raises(ValueError, match="must be on cuda:0"):
         D.t2part_device(torch.zeros(V * 40, dtype=torch.float64), po, parts)
+
+
+def test_request_arrays_refuse_mismatched_host_arrays(dmi):
+    """The request loop's host arrays: the library reads V x nTE image doubles and V B1-map doubles, so a B1 map that
+    numpy would broadcast (a scalar, one slice) is refused rather than passed with its own, shorter buffer."""
+    import numpy as np
+
+    from decaes_amd.server import request_arrays
+
+    o = _opts(dmi, SaveRegParam=True)
+    img = np.zeros((4, 3, 2, 32))
+    with pytest.raises(AssertionError, match="image shape"):
+        request_arrays(np.zeros((4, 3, 2, 31)), o)
+    for bad in (np.float64(150.0), np.full((4, 3, 1), 150.0), np.full((2, 3, 4), 150.0)):
+        with pytest.raises(AssertionError, match="B1 map shape"):
+            request_arrays(img, o, bad)
+    b1 = np.random.default_rng(0).uniform(130, 180, (4, 3, 2))
+    image, alpha, out, dist = request_arrays(img, o, b1)
+    assert image.flags.f_contiguous and alpha.flags.f_contiguous
+    assert np.array_equal(out["alpha"], b1) and np.isnan(out["mu"]).all() and dist.shape == (4, 3, 2, 40)
