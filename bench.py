@@ -20,7 +20,6 @@ stream below capacity, the roofline object and the CPU baseline.
 from __future__ import annotations
 
 import argparse
-import dataclasses
 import json
 import os
 import sys

@@ -1,7 +1,5 @@
 """CPU: the command line layer's file formats and argument handling (src/main.jl; test/cli.jl's conventions)."""
-import gzip
 import struct
-import warnings
 from pathlib import Path
 
 import numpy as np

@@ -4,7 +4,6 @@ save_results! closed forms, threshold semantics and T2part window arithmetic on 
 import math
 
 import numpy as np
-import pytest
 
 
 def _epg_image(oracle, alphas, T2s, fracs, m=32, TE=10e-3):
