@@ -1481,7 +1481,7 @@ __device__ __forceinline__ void fprota(double cs, double sn, double &a, double &
 // its row with readlane, computes in all lanes alike, and the owner keeps the result.
 __device__ int spline_fit(Wave &W, double xv, double yv, int np, int &k_out) {
     const int L = W.L;
-    const int k = min(3, np - 1), k1 = k + 1, k2 = k + 2, n = np + k1, nk1 = np, k3 = k / 2;
+    const int k = min(3, np - 1), k1 = k + 1, n = np + k1, nk1 = np, k3 = k / 2;
     ldbl *t1 = W.spt - 1;
     // knots: t(1..k1) = x(1), t(n-k..n) = x(np); t(i) for k2 <= i <= n - k1: x(j) (odd k) or (x(j) + x(j-1)) / 2
     // (even k), j = i - k + k3
@@ -2186,9 +2186,7 @@ struct LCurve {
         __syncthreads();
         const double Ptl[2] = {Ps[0][0], Ps[0][1]}, Pbr[2] = {Ps[3][0], Ps[3][1]};
         update_curvature(xs, Ps, Ptl, Pbr, Ctol);
-        int iter = 0;
         while (!overflow && !(fabs(xs[3] - xs[0]) < xtol || norm2(Ps[0][0] - Ps[3][0], Ps[0][1] - Ps[3][1]) < Ptol)) {
-            iter += 1;
             // backtracking keeps the current state (see above)
             int i2 = find(pcx, npc, xs[1]), i3 = find(pcx, npc, xs[2]);
             double C2 = rfl(pcC[i2]), C3 = rfl(pcC[i3]);
