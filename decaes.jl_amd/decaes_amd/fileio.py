@@ -66,7 +66,11 @@ def read_nifti(path) -> np.ndarray:
     data = np.frombuffer(raw, dtype=dt, count=count, offset=off).reshape(shape, order="F")
     if scl_slope == 0 or not np.isfinite(scl_slope):
         scl_slope, scl_inter = 1.0, 0.0
-    return np.asfortranarray(data.astype(np.float64) * scl_slope + scl_inter)
+    # data * scl_slope + scl_inter in FP64 into one column-major array: no full-size temporaries
+    out = np.empty(shape, dtype=np.float64, order="F")
+    np.multiply(data, scl_slope, out=out, dtype=np.float64)
+    np.add(out, scl_inter, out=out)
+    return out
 
 
 def write_nifti(path, data: np.ndarray, pixdim=(1.0, 1.0, 1.0)) -> None:
